@@ -1,0 +1,135 @@
+"""GPU parity of the Hamming matcher: exact (query, index, distance) sets.
+
+Oracles: the C brute force (oracle/pdq_oracle.c) at small sizes, numpy
+bitwise_count brute force at larger sizes, and the reference's planted-control
+protocol (src/matching/mod.rs:460-538).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from yume_pdq import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def as_triples(hits):
+    return np.stack([hits["query"].astype(np.int64), hits["index"].astype(np.int64),
+                     hits["distance"].astype(np.int64)], axis=1) if hits.size else np.zeros((0, 3), np.int64)
+
+
+def sort_triples(t):
+    if t.shape[0] == 0:
+        return t
+    return t[np.lexsort((t[:, 0], t[:, 1]))]
+
+
+@pytest.mark.parametrize("nq", [1, 8, 24, 64])
+def test_planted_controls(gpu, oracle, dev, nq):
+    """Reference protocol: plant a control with 31 flips (with replacement, so
+    distance <= 31) at each position for each needle; find must report it."""
+    import torch
+    rng = np.random.default_rng(int.from_bytes(b"yume-pdq", "big") % (2 ** 63))
+    hay = rng.integers(0, 256, (2048, 32), dtype=np.uint8)
+    needles = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    d_hay = torch.from_numpy(hay).to(dev)
+    say(f"[match] planted controls nq={nq}: negative control")
+    total, hits = gpu.match(d_hay, torch.from_numpy(needles).to(dev), 31)
+    assert total == 0
+    assert gpu.match_first(d_hay, torch.from_numpy(needles).to(dev), 31) is None
+    positions = rng.choice(2048, size=64, replace=False)
+    for pos in positions:
+        i = int(rng.integers(0, nq))
+        nd = needles.copy()
+        nd[i] = hay[pos]
+        for _ in range(31):
+            b = int(rng.integers(0, 32))
+            nd[i, b] ^= np.uint8(1 << int(rng.integers(0, 8)))
+        d_nd = torch.from_numpy(nd).to(dev)
+        total, hits = gpu.match(d_hay, d_nd, 31)
+        exp_n, exp = oracle.match_all(hay, nd, 31)
+        assert total == exp_n >= 1
+        assert np.array_equal(sort_triples(as_triples(hits)), exp)
+        first = gpu.match_first(d_hay, d_nd, 31)
+        assert first == oracle.match_first(hay, nd, 31)
+        assert first == (int(pos), i)
+
+
+@pytest.mark.parametrize("threshold", [0, 1, 31, 64, 128, 200, 255, 256])
+def test_thresholds(gpu, oracle, dev, threshold):
+    import torch
+    db = synthetic.random_db(3000, seed=3)
+    q = synthetic.random_db(8, seed=4)
+    rng = np.random.default_rng(9)
+    synthetic.plant(db, q, 200, seed=5, max_dist=60)
+    say(f"[match] threshold={threshold}")
+    exp_n, exp = oracle.match_all(db, q, threshold, cap=3000 * 8)
+    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), threshold,
+                            capacity=3000 * 8)
+    assert total == exp_n
+    assert np.array_equal(sort_triples(as_triples(hits)), exp)
+    del rng
+
+
+def test_empty_and_tiny(gpu, dev):
+    import torch
+    q = torch.from_numpy(synthetic.random_db(8, seed=1)).to(dev)
+    say("[match] empty db")
+    empty = torch.empty((0, 32), dtype=torch.uint8, device=dev)
+    assert gpu.match(empty, q, 31)[0] == 0
+    assert gpu.match_first(empty, q, 31) is None
+    say("[match] db of 1 row equal to a query")
+    one = q[3:4].clone()
+    total, hits = gpu.match(one, q, 0)
+    assert total >= 1 and (hits["query"] == 3).any() and (hits["distance"][hits["query"] == 3] == 0).all()
+
+
+def test_overflow_reports_exact_count(gpu, oracle, dev):
+    """More hits than capacity: the count stays exact, the stored hits are a subset."""
+    import torch
+    db = synthetic.random_db(5000, seed=21)
+    q = synthetic.random_db(4, seed=22)
+    say("[match] overflow, threshold 256 (every pair)")
+    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 256, capacity=1000)
+    assert total == 5000 * 4
+    assert hits.size == 1000
+    assert len(set(zip(hits["query"].tolist(), hits["index"].tolist()))) == 1000
+
+
+def test_index_base_and_append(gpu, oracle, dev):
+    """Two shards into one hit buffer with global indices (sharded DB)."""
+    import torch
+    db = synthetic.random_db(4000, seed=31)
+    q = synthetic.random_db(8, seed=32)
+    synthetic.plant(db, q, 100, seed=33)
+    d_db = torch.from_numpy(db).to(dev)
+    d_q = torch.from_numpy(q).to(dev)
+    say("[match] two shards, append")
+    hits, count = gpu.match(d_db[:1500], d_q, 31, capacity=4096, sync=False)
+    gpu.match(d_db[1500:], d_q, 31, index_base=1500, hits=hits, count=count, append=True, sync=False)
+    total = int(count.item())
+    arr = np.sort(hits[:total].cpu().numpy().copy().view(gpu.HIT_DTYPE).reshape(-1), order=("index", "query"))
+    exp_n, exp = oracle.match_all(db, q, 31)
+    assert total == exp_n
+    assert np.array_equal(as_triples(arr), exp)
+
+
+@pytest.mark.parametrize("n_rows,nq", [(1_000_000, 8), (1_000_000, 128), (10_000_000, 8)])
+def test_scale_exact_set(gpu, dev, n_rows, nq):
+    """Config-sized DBs with planted near-duplicates: exact set vs numpy brute force."""
+    import torch
+    db = synthetic.random_db(n_rows, seed=7)
+    q = synthetic.random_db(nq, seed=8)
+    synthetic.plant(db, q, max(1000, n_rows // 200), seed=12)
+    say(f"[match] scale n_rows={n_rows} nq={nq}")
+    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 31,
+                            capacity=1 << 20)
+    exp = synthetic.brute_force_hits(db, q, 31)
+    assert total == exp.shape[0]
+    assert np.array_equal(as_triples(hits), exp)

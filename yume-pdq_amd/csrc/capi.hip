@@ -1,0 +1,288 @@
+// C ABI of libyume_pdq.so: the drop-in entry points of the reference's
+// include/yume_pdq.h and the batched device API of include/yume_pdq_amd.h.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/yume_pdq.h"
+#include "../../include/yume_pdq_amd.h"
+#include "pdq_device.h"
+
+// ---------------------------------------------------------------------------
+// Data symbols of the reference ABI.
+// ---------------------------------------------------------------------------
+// Declared outside any extern "C" block, as in the reference header; global
+// variables are not name-mangled, so C and C++ callers see the same symbols.
+const uint32_t YUME_PDQ_VERSION_MAJOR = 1;
+const uint32_t YUME_PDQ_VERSION_MINOR = 1;
+const uint32_t YUME_PDQ_VERSION_PATCH = 0;
+const char YUME_PDQ_VERSION_STRING[] = "1.1.0";
+
+extern "C" {
+
+// Dihedrals::from_tuples packs (dx0, dx1, dy0, dy1) big-endian (src/lib.rs:96-133).
+#define YPDQ_PACK(a, b, c, d) \
+  ((uint32_t)(uint8_t)(a) << 24 | (uint32_t)(uint8_t)(b) << 16 | (uint32_t)(uint8_t)(c) << 8 | (uint32_t)(uint8_t)(d))
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_NORMAL = {YPDQ_PACK(1, 0, 0, 1)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_FLIPPED = {YPDQ_PACK(1, 0, 0, -1)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_FLOPPED = {YPDQ_PACK(-1, 0, 0, 1)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_ROTATED_180 = {YPDQ_PACK(-1, 0, 0, -1)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_ROTATED_90 = {YPDQ_PACK(0, 1, -1, 0)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_ROTATED_270 = {YPDQ_PACK(0, 1, 1, 0)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_ROTATED_90_FLOPPED = {YPDQ_PACK(0, -1, -1, 0)};
+const struct YumePDQ_Dihedrals YUME_PDQ_DIHEDRAL_FLOPPED_ROTATED_270 = {YPDQ_PACK(0, -1, 1, 0)};
+#undef YPDQ_PACK
+}
+
+namespace {
+
+using ypdq::HashOutputs;
+
+// Device-side result block of one drop-in call, copied back in one transfer.
+struct DropInResult {
+  uint8_t hashes[8 * 32];
+  float quality[8];
+  float threshold[8];
+  float pdqf[256];
+  float tmp[128];
+  float buf1[128 * 128];
+  float dihedral_pdqf[8 * 256];
+};
+
+// Per-thread device state for the drop-in entry points (the reference clones
+// its kernel per call, src/lib.rs:266, so calls are reentrant).
+struct DropInState {
+  hipStream_t stream = nullptr;
+  float* d_in = nullptr;  // 512 x 512 f32, or 256 PDQF + 1 threshold
+  DropInResult* d_res = nullptr;
+  DropInResult* h_res = nullptr;  // pinned
+};
+
+std::mutex g_pool_mu;
+std::vector<DropInState*> g_pool;
+
+[[noreturn]] void die(const char* what, hipError_t e) {
+  fprintf(stderr, "libyume_pdq: %s failed: %s\n", what, hipGetErrorString(e));
+  abort();
+}
+
+DropInState* state_new() {
+  DropInState* s = new DropInState();
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) die("hipStreamCreate", e);
+  if ((e = hipMalloc(&s->d_in, 512 * 512 * sizeof(float))) != hipSuccess) die("hipMalloc", e);
+  if ((e = hipMalloc(&s->d_res, sizeof(DropInResult))) != hipSuccess) die("hipMalloc", e);
+  if ((e = hipHostMalloc(&s->h_res, sizeof(DropInResult), hipHostMallocDefault)) != hipSuccess)
+    die("hipHostMalloc", e);
+  return s;
+}
+
+// Returns the state to a process-wide pool when its thread exits (no HIP calls
+// in thread teardown); states are reused by later threads.
+struct StateHolder {
+  DropInState* s = nullptr;
+  ~StateHolder() {
+    if (s) {
+      std::lock_guard<std::mutex> g(g_pool_mu);
+      g_pool.push_back(s);
+    }
+  }
+};
+
+DropInState* state() {
+  static thread_local StateHolder h;
+  if (!h.s) {
+    {
+      std::lock_guard<std::mutex> g(g_pool_mu);
+      if (!g_pool.empty()) {
+        h.s = g_pool.back();
+        g_pool.pop_back();
+      }
+    }
+    if (!h.s) h.s = state_new();
+  }
+  return h.s;
+}
+
+HashOutputs outputs_of(DropInResult* r, bool with_buf1) {
+  HashOutputs o;
+  o.hashes = r->hashes;
+  o.quality = r->quality;
+  o.threshold = r->threshold;
+  o.pdqf = r->pdqf;
+  o.buf1 = with_buf1 ? r->buf1 : nullptr;
+  o.tmp = with_buf1 ? r->tmp : nullptr;
+  o.dihedral_pdqf = r->dihedral_pdqf;
+  return o;
+}
+
+int from_hip(hipError_t e) {
+  if (e == hipSuccess) return YUME_PDQ_AMD_OK;
+  if (e == hipErrorOutOfMemory) return YUME_PDQ_AMD_ENOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return YUME_PDQ_AMD_ENODEV;
+  return YUME_PDQ_AMD_EHIP;
+}
+
+__global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+}  // namespace
+
+extern "C" {
+
+float yume_pdq_hash_smart_kernel(const float* input, float* threshold, uint8_t* output, float* buf1, float* tmp,
+                                 float* pdqf) {
+  DropInState* s = state();
+  hipError_t e;
+  if ((e = hipMemcpyAsync(s->d_in, input, 512 * 512 * sizeof(float), hipMemcpyHostToDevice, s->stream)) !=
+      hipSuccess)
+    die("hipMemcpyAsync", e);
+  HashOutputs o = outputs_of(s->d_res, true);
+  o.dihedral_pdqf = nullptr;
+  if ((e = ypdq::launch_hash(s->d_in, ypdq::kInputF32, 512 * 512 * sizeof(float), 1, ypdq::kDihedralAvx2, o,
+                             s->stream)) != hipSuccess)
+    die("hash kernel launch", e);
+  if ((e = hipMemcpyAsync(s->h_res, s->d_res, offsetof(DropInResult, dihedral_pdqf), hipMemcpyDeviceToHost,
+                          s->stream)) != hipSuccess)
+    die("hipMemcpyAsync", e);
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) die("hipStreamSynchronize", e);
+  const DropInResult* r = s->h_res;
+  memcpy(output, r->hashes, 32);
+  if (threshold) *threshold = r->threshold[0];
+  memcpy(buf1, r->buf1, sizeof r->buf1);
+  memcpy(tmp, r->tmp, sizeof r->tmp);
+  memcpy(pdqf, r->pdqf, sizeof r->pdqf);
+  return r->quality[0];
+}
+
+bool yume_pdq_visit_dihedrals_smart_kernel(void* ctx, float threshold, uint8_t* output, float* pdqf,
+                                           DihedralCallback callback) {
+  DropInState* s = state();
+  hipError_t e;
+  float* h_in = reinterpret_cast<float*>(s->h_res);  // staging: 256 PDQF + threshold
+  memcpy(h_in, pdqf, 256 * sizeof(float));
+  h_in[256] = threshold;
+  if ((e = hipMemcpyAsync(s->d_in, h_in, 257 * sizeof(float), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
+    die("hipMemcpyAsync", e);
+  // the staging area is overwritten by the result copy below, after the upload
+  HashOutputs o = outputs_of(s->d_res, false);
+  o.pdqf = nullptr;
+  if ((e = ypdq::launch_dihedrals(s->d_in, s->d_in + 256, 1, ypdq::kDihedralAvx2, o, s->stream)) != hipSuccess)
+    die("dihedral kernel launch", e);
+  if ((e = hipMemcpyAsync(s->h_res, s->d_res, sizeof(DropInResult), hipMemcpyDeviceToHost, s->stream)) !=
+      hipSuccess)
+    die("hipMemcpyAsync", e);
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) die("hipStreamSynchronize", e);
+  const DropInResult* r = s->h_res;
+  // orientations 1..7 are the reference's visiting order (src/lib.rs:389-418)
+  static const uint32_t codes[8] = {
+      0, YUME_PDQ_DIHEDRAL_FLOPPED.packed, YUME_PDQ_DIHEDRAL_ROTATED_180.packed, YUME_PDQ_DIHEDRAL_FLIPPED.packed,
+      YUME_PDQ_DIHEDRAL_ROTATED_90_FLOPPED.packed, YUME_PDQ_DIHEDRAL_ROTATED_270.packed,
+      YUME_PDQ_DIHEDRAL_FLOPPED_ROTATED_270.packed, YUME_PDQ_DIHEDRAL_ROTATED_90.packed};
+  for (int d = 1; d < 8; d++) {
+    memcpy(output, r->hashes + 32 * d, 32);
+    memcpy(pdqf, r->dihedral_pdqf + 256 * d, 256 * sizeof(float));
+    if (!callback(ctx, codes[d], threshold, r->quality[d])) return false;
+  }
+  return true;
+}
+
+const char* yume_pdq_amd_strerror(int code) {
+  switch (code) {
+    case YUME_PDQ_AMD_OK: return "ok";
+    case YUME_PDQ_AMD_EINVAL: return "invalid argument";
+    case YUME_PDQ_AMD_EHIP: return "HIP runtime error";
+    case YUME_PDQ_AMD_ENOMEM: return "device out of memory";
+    case YUME_PDQ_AMD_ENODEV: return "no usable GPU";
+    case YUME_PDQ_AMD_EOVERFLOW: return "hit buffer overflow";
+    default: return "unknown error";
+  }
+}
+
+int yume_pdq_amd_device_info(int device, char* name, size_t name_len, int* compute_units) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return YUME_PDQ_AMD_ENODEV;
+  if (device < 0 || device >= n) return YUME_PDQ_AMD_EINVAL;
+  hipDeviceProp_t p;
+  hipError_t e = hipGetDeviceProperties(&p, device);
+  if (e != hipSuccess) return from_hip(e);
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (compute_units) *compute_units = p.multiProcessorCount;
+  return n;
+}
+
+int yume_pdq_amd_hash_batch(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
+                            uint32_t dihedral_mode, uint8_t* d_hashes, float* d_quality, float* d_threshold,
+                            float* d_pdqf, void* stream) {
+  if (n_images == 0) return YUME_PDQ_AMD_OK;
+  if (!d_planes || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
+  if (input_kind != YUME_PDQ_AMD_INPUT_F32 && input_kind != YUME_PDQ_AMD_INPUT_U8) return YUME_PDQ_AMD_EINVAL;
+  if (dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = 512ull * 512 * (input_kind == YUME_PDQ_AMD_INPUT_F32 ? 4 : 1);
+  if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
+    return YUME_PDQ_AMD_EINVAL;
+  HashOutputs o;
+  o.hashes = d_hashes;
+  o.quality = d_quality;
+  o.threshold = d_threshold;
+  o.pdqf = d_pdqf;
+  o.buf1 = nullptr;
+  o.tmp = nullptr;
+  o.dihedral_pdqf = nullptr;
+  return from_hip(ypdq::launch_hash(d_planes, input_kind, plane_stride_bytes, n_images, dihedral_mode, o,
+                                    static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_dihedrals_batch(const float* d_pdqf, const float* d_thr_in, uint32_t n, uint32_t dihedral_mode,
+                                 uint8_t* d_hashes, float* d_quality, float* d_threshold, float* d_dihedral_pdqf,
+                                 void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pdqf || !d_thr_in || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
+  if (dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
+  HashOutputs o;
+  o.hashes = d_hashes;
+  o.quality = d_quality;
+  o.threshold = d_threshold;
+  o.pdqf = nullptr;
+  o.buf1 = nullptr;
+  o.tmp = nullptr;
+  o.dihedral_pdqf = d_dihedral_pdqf;
+  return from_hip(ypdq::launch_dihedrals(d_pdqf, d_thr_in, n, dihedral_mode, o, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,
+                       uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity, unsigned long long* d_count,
+                       uint64_t index_base, uint32_t flags, void* stream) {
+  if (!d_count || (capacity && !d_hits)) return YUME_PDQ_AMD_EINVAL;
+  if ((n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;
+  if ((uintptr_t)d_db % 16 || (uintptr_t)d_queries % 4) return YUME_PDQ_AMD_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!(flags & YUME_PDQ_AMD_MATCH_APPEND)) {
+    // a kernel, not hipMemsetAsync, so the sequence can be graph-captured
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, st, d_count, 0ull);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return from_hip(e);
+  }
+  static_assert(sizeof(YumePdqAmdHit) == sizeof(ypdq::MatchHit), "hit layout");
+  return from_hip(ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold,
+                                     reinterpret_cast<ypdq::MatchHit*>(d_hits), capacity, d_count, index_base, st));
+}
+
+int yume_pdq_amd_match_first(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,
+                             uint32_t threshold, unsigned long long* d_first, void* stream) {
+  if (!d_first || (n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;
+  if ((uintptr_t)d_db % 16 || (uintptr_t)d_queries % 4) return YUME_PDQ_AMD_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, st, d_first, ~0ull);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return from_hip(e);
+  return from_hip(ypdq::launch_match_first(d_db, n_rows, d_queries, n_queries, threshold, d_first, st));
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Device-side helpers and the kernel/host interface shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ypdq {
+
+constexpr uint32_t kDihedralExact = 0;  // sign algebra over all 16 columns
+constexpr uint32_t kDihedralAvx2 = 1;   // the AVX2 kernel's column extent (x86.rs:330-376)
+constexpr int kInputF32 = 0;
+constexpr int kInputU8 = 1;
+
+// Per-image outputs of the hash kernels; every array is indexed by image.
+struct HashOutputs {
+  uint8_t* hashes;   // [n][8][32] NORMAL, FLOPPED, ROTATED_180, FLIPPED,
+                     // ROTATED_90_FLOPPED, ROTATED_270, FLOPPED_ROTATED_270, ROTATED_90
+  float* quality;    // [n][8]
+  float* threshold;  // [n][8]
+  float* pdqf;       // [n][256] NORMAL PDQF, or null
+  float* buf1;       // [n][128*128] compressed plane (drop-in only), or null
+  float* tmp;        // [n][128] last stage-1 row (drop-in only), or null
+  float* dihedral_pdqf;  // [n][8][256] every orientation's PDQF, or null
+};
+
+struct MatchHit {
+  uint32_t query;
+  uint32_t distance;
+  uint64_t index;
+};
+
+__device__ __forceinline__ float shfl_down1(float x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ float shfl_xor(float x, int m) { return __shfl_xor(x, m, 64); }
+__device__ __forceinline__ uint32_t shfl_xor_u(uint32_t x, int m) {
+  return static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), m, 64));
+}
+__device__ __forceinline__ float shfl_idx(float x, int src) { return __shfl(x, src, 64); }
+
+// Make LDS writes of this wavefront visible to its other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool wave_any(bool b) { return __any(b ? 1 : 0) != 0; }
+__device__ __forceinline__ bool wave_all(bool b) { return __all(b ? 1 : 0) != 0; }
+
+hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
+                       const HashOutputs& out, hipStream_t stream);
+hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, uint32_t mode,
+                            const HashOutputs& out, hipStream_t stream);
+hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
+                        MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
+                        hipStream_t stream);
+hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
+                              uint32_t threshold, unsigned long long* first_key, hipStream_t stream);
+
+}  // namespace ypdq

@@ -1,0 +1,136 @@
+// Exhaustive Hamming matcher for gfx950: every (query, row) pair with
+// popcount(row ^ query) <= threshold over 256-bit PDQ hashes.
+//
+// Semantics follow the reference's matchers: CpuMatcher::scan/find
+// (src/matching/mod.rs:232-257, 336-362) and VulkanMatcher
+// (src/matching/vulkan.rs:111-146): inclusive threshold, distance = sum of
+// popcounts over the 32-byte row.  The reference's matchers report only
+// (row, needle); this one reports (query, index, distance) triples and also a
+// first-hit mode equal to the lexicographically smallest (row, needle), which
+// is what VulkanMatcher::find returns (atomicMin of needle + row * N).
+//
+// Layout: the database is resident in HBM as 32-byte rows (the hash bytes in
+// the order the hash kernel writes them).  Each lane streams whole rows with
+// two 16-byte loads; queries are wave-uniform and come through the scalar
+// cache into SGPRs.  Per (row, query) the first 128 bits are compared first:
+// their distance is a lower bound of the full distance, so a row whose
+// 128-bit distance already exceeds the threshold is rejected exactly.  For
+// random rows that happens for all but ~1e-9 of pairs, so the second half is
+// almost never touched by the ALU (it still arrives with the same cache line).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pdq_device.h"
+
+namespace ypdq {
+
+constexpr int kMatchBlock = 256;
+constexpr int kRowsPerLane = 2;
+
+// popcount(x) + acc: one v_bcnt_u32_b32
+__device__ __forceinline__ uint32_t popc(uint32_t x, uint32_t acc) { return (uint32_t)__builtin_popcount(x) + acc; }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Database rows are streamed once per scan: non-temporal loads.
+__device__ __forceinline__ uint4 load_nt(const uint4* p) {
+  const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+struct Row256 {
+  uint4 lo, hi;
+};
+
+__device__ __forceinline__ uint32_t dist128(const uint4& r, const uint32_t* q) {
+  uint32_t d = popc(r.x ^ q[0], 0u);
+  d = popc(r.y ^ q[1], d);
+  d = popc(r.z ^ q[2], d);
+  return popc(r.w ^ q[3], d);
+}
+
+template <bool kFirst>
+__global__ void __launch_bounds__(kMatchBlock) pdq_match_kernel(const uint4* __restrict__ db, uint64_t n_rows,
+                                                                const uint32_t* __restrict__ queries,
+                                                                uint32_t n_queries, uint32_t threshold,
+                                                                MatchHit* __restrict__ hits, uint64_t capacity,
+                                                                unsigned long long* __restrict__ counter,
+                                                                uint64_t index_base) {
+  const uint64_t stride = (uint64_t)gridDim.x * kMatchBlock * kRowsPerLane;
+  for (uint64_t base = ((uint64_t)blockIdx.x * kMatchBlock * kRowsPerLane) + threadIdx.x; base < n_rows;
+       base += stride) {
+    Row256 row[kRowsPerLane];
+    bool valid[kRowsPerLane];
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; u++) {
+      const uint64_t r = base + (uint64_t)u * kMatchBlock;
+      valid[u] = r < n_rows;
+      if (valid[u]) {
+        row[u].lo = load_nt(db + 2 * r);
+        row[u].hi = load_nt(db + 2 * r + 1);
+      } else {
+        row[u].lo = make_uint4(0, 0, 0, 0);
+        row[u].hi = make_uint4(0, 0, 0, 0);
+      }
+    }
+    for (uint32_t qi = 0; qi < n_queries; qi++) {
+      const uint32_t* q = queries + 8 * qi;
+#pragma unroll
+      for (int u = 0; u < kRowsPerLane; u++) {
+        const uint32_t d0 = dist128(row[u].lo, q);
+        if (__builtin_expect(d0 <= threshold && valid[u], 0)) {
+          const uint32_t d = d0 + dist128(row[u].hi, q + 4);
+          if (d <= threshold) {
+            const uint64_t r = base + (uint64_t)u * kMatchBlock;
+            if (kFirst) {
+              atomicMin(counter, (unsigned long long)(r * n_queries + qi));
+            } else {
+              const unsigned long long slot = atomicAdd(counter, 1ull);
+              if (slot < capacity) {
+                MatchHit h;
+                h.query = qi;
+                h.distance = d;
+                h.index = index_base + r;
+                hits[slot] = h;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+static int match_grid(uint64_t n_rows) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+  }
+  const uint64_t per_block = (uint64_t)kMatchBlock * kRowsPerLane;
+  uint64_t need = (n_rows + per_block - 1) / per_block;
+  uint64_t cap = (uint64_t)cus * 16;  // enough waves in flight to cover HBM latency
+  if (need > cap) need = cap;
+  return (int)(need ? need : 1);
+}
+
+hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
+                        MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
+                        hipStream_t stream) {
+  if (n_rows == 0 || n_queries == 0) return hipSuccess;
+  hipLaunchKernelGGL(pdq_match_kernel<false>, dim3(match_grid(n_rows)), dim3(kMatchBlock), 0, stream,
+                     static_cast<const uint4*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
+                     threshold, hits, capacity, counter, index_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
+                              uint32_t threshold, unsigned long long* first_key, hipStream_t stream) {
+  if (n_rows == 0 || n_queries == 0) return hipSuccess;
+  hipLaunchKernelGGL(pdq_match_kernel<true>, dim3(match_grid(n_rows)), dim3(kMatchBlock), 0, stream,
+                     static_cast<const uint4*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
+                     threshold, nullptr, 0, first_key, 0);
+  return hipGetLastError();
+}
+
+}  // namespace ypdq

@@ -1,0 +1,255 @@
+"""Python binding of libyume_pdq.so (MI355X / gfx950 PDQ hash + Hamming matcher).
+
+Mirrors the reference's FFI (include/yume_pdq.h, integration/hash.py) and adds
+the batched device API of include/yume_pdq_amd.h over torch device tensors.
+Every call runs on the GPU through the HIP library; if the library is missing
+the import of the binding raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("LIBYUME_PDQ_PATH", os.path.join(PKG_DIR, "libyume_pdq.so"))
+
+INPUT_F32, INPUT_U8 = 0, 1
+DIHEDRAL_EXACT, DIHEDRAL_AVX2 = 0, 1
+MATCH_APPEND = 1
+
+# Orientation order of the 8 hashes per image (YUME_PDQ_AMD_D_*).
+DIHEDRALS = ["NORMAL", "FLOPPED", "ROTATED_180", "FLIPPED", "ROTATED_90_FLOPPED",
+             "ROTATED_270", "FLOPPED_ROTATED_270", "ROTATED_90"]
+
+DIHEDRAL_CODES = {  # Dihedrals::* packed, src/lib.rs:119-133
+    "NORMAL": 0x01000001, "FLIPPED": 0x010000FF, "FLOPPED": 0xFF000001,
+    "ROTATED_180": 0xFF0000FF, "ROTATED_90": 0x0001FF00, "ROTATED_270": 0x00010100,
+    "ROTATED_90_FLOPPED": 0x00FFFF00, "FLOPPED_ROTATED_270": 0x00FF0100,
+}
+
+
+class YumePdqError(RuntimeError):
+    pass
+
+
+class Hit(C.Structure):
+    _fields_ = [("query", C.c_uint32), ("distance", C.c_uint32), ("index", C.c_uint64)]
+
+
+DihedralCallback = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_uint32, C.c_float, C.c_float)
+
+_lib = None
+
+
+def lib():
+    """Load libyume_pdq.so (built by __graft_entry__.build() / make -C yume-pdq_amd)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise YumePdqError(f"libyume_pdq.so not found at {LIB_PATH}; build it first "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        vp, f, u32, u64, i = C.c_void_p, C.c_float, C.c_uint32, C.c_uint64, C.c_int
+        L.yume_pdq_hash_smart_kernel.restype = f
+        L.yume_pdq_hash_smart_kernel.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.yume_pdq_visit_dihedrals_smart_kernel.restype = C.c_bool
+        L.yume_pdq_visit_dihedrals_smart_kernel.argtypes = [vp, f, vp, vp, DihedralCallback]
+        L.yume_pdq_amd_strerror.restype = C.c_char_p
+        L.yume_pdq_amd_strerror.argtypes = [i]
+        L.yume_pdq_amd_device_info.restype = i
+        L.yume_pdq_amd_device_info.argtypes = [i, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+        L.yume_pdq_amd_hash_batch.restype = i
+        L.yume_pdq_amd_hash_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_dihedrals_batch.restype = i
+        L.yume_pdq_amd_dihedrals_batch.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_match.restype = i
+        L.yume_pdq_amd_match.argtypes = [vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
+        L.yume_pdq_amd_match_first.restype = i
+        L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise YumePdqError(f"{what}: {lib().yume_pdq_amd_strerror(rc).decode()} ({rc})")
+    return rc
+
+
+def version():
+    L = lib()
+    return tuple(C.c_uint32.in_dll(L, n).value for n in
+                 ("YUME_PDQ_VERSION_MAJOR", "YUME_PDQ_VERSION_MINOR", "YUME_PDQ_VERSION_PATCH"))
+
+
+def dihedral_constant(name):
+    return C.c_uint32.in_dll(lib(), "YUME_PDQ_DIHEDRAL_" + name).value
+
+
+def device_info(device=0):
+    name = C.create_string_buffer(256)
+    cus = C.c_int(0)
+    n = _check(lib().yume_pdq_amd_device_info(device, name, 256, C.byref(cus)), "device_info")
+    return {"devices": n, "name": name.value.decode(), "compute_units": cus.value}
+
+
+# ---------------------------------------------------------------------------
+# Drop-in ABI (the reference's include/yume_pdq.h)
+# ---------------------------------------------------------------------------
+def hash_smart_kernel(image):
+    """yume_pdq_hash_smart_kernel on one 512x512 f32 image (host arrays).
+
+    Returns (hash[32] u8, threshold, quality, pdqf[16,16], buf1[128,128], tmp[128])."""
+    a = np.ascontiguousarray(image, dtype=np.float32)
+    if a.shape != (512, 512):
+        raise ValueError("image must be 512x512")
+    thr = np.zeros(1, np.float32)
+    out = np.zeros(32, np.uint8)
+    buf1 = np.zeros((128, 128), np.float32)
+    tmp = np.zeros(128, np.float32)
+    pdqf = np.zeros((16, 16), np.float32)
+    p = lambda x: x.ctypes.data_as(C.c_void_p)  # noqa: E731
+    q = lib().yume_pdq_hash_smart_kernel(p(a), p(thr), p(out), p(buf1), p(tmp), p(pdqf))
+    return out, float(thr[0]), float(q), pdqf, buf1, tmp
+
+
+def visit_dihedrals_smart_kernel(threshold, pdqf, callback):
+    """yume_pdq_visit_dihedrals_smart_kernel.  callback(code, threshold, quality,
+    hash[32], pdqf[16,16]) -> bool (False stops).  Returns the ABI's bool."""
+    out = np.zeros(32, np.uint8)
+    pd = np.ascontiguousarray(pdqf, dtype=np.float32).copy()
+
+    def tramp(_ctx, code, thr, quality):
+        return bool(callback(code, thr, quality, out.copy(), pd.reshape(16, 16).copy()))
+
+    cb = DihedralCallback(tramp)
+    return bool(lib().yume_pdq_visit_dihedrals_smart_kernel(
+        None, C.c_float(threshold), out.ctypes.data_as(C.c_void_p), pd.ctypes.data_as(C.c_void_p), cb))
+
+
+# ---------------------------------------------------------------------------
+# Batched device API over torch tensors (include/yume_pdq_amd.h)
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None):
+    """Hash n 512x512 planes resident on the GPU (torch f32 or u8, shape (n,512,512)).
+
+    Returns (hashes (n,8,32) u8, quality (n,8) f32, threshold (n,8) f32[, pdqf (n,16,16)])."""
+    torch = _torch()
+    if planes.dim() == 2:
+        planes = planes.unsqueeze(0)
+    if planes.shape[-2:] != (512, 512) or not planes.is_cuda:
+        raise ValueError("planes must be a CUDA tensor of shape (n, 512, 512)")
+    if planes.dtype == torch.float32:
+        kind, elem = INPUT_F32, 4
+    elif planes.dtype == torch.uint8:
+        kind, elem = INPUT_U8, 1
+    else:
+        raise ValueError("planes must be float32 or uint8")
+    if planes.stride(-1) != 1 or planes.stride(-2) != 512:
+        planes = planes.contiguous()
+    n = planes.shape[0]
+    # a size-1 batch dimension may carry any stride (e.g. 0 from x[None])
+    stride = planes.stride(0) if n > 1 else 512 * 512
+    if stride < 512 * 512:
+        planes = planes.contiguous()
+        stride = 512 * 512
+    dev = planes.device
+    if out is None:
+        hashes = torch.empty((n, 8, 32), dtype=torch.uint8, device=dev)
+        quality = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        threshold = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        pdqf = torch.empty((n, 16, 16), dtype=torch.float32, device=dev) if want_pdqf else None
+    else:
+        hashes, quality, threshold, pdqf = out
+    _check(lib().yume_pdq_amd_hash_batch(
+        C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode,
+        C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+        C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
+        _stream_ptr(stream)), "hash_batch")
+    if want_pdqf:
+        return hashes, quality, threshold, pdqf
+    return hashes, quality, threshold
+
+
+def dihedrals_batch(pdqf, thr_in, dihedral_mode=DIHEDRAL_EXACT, stream=None):
+    """visit_dihedrals on device PDQFs (n,16,16) with thresholds (n,): returns
+    (hashes (n,8,32), quality (n,8), threshold (n,8), pdqfs (n,8,16,16))."""
+    torch = _torch()
+    pdqf = pdqf.contiguous().to(torch.float32)
+    thr_in = thr_in.contiguous().to(torch.float32)
+    n = pdqf.shape[0]
+    dev = pdqf.device
+    hashes = torch.empty((n, 8, 32), dtype=torch.uint8, device=dev)
+    quality = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    threshold = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    pds = torch.empty((n, 8, 16, 16), dtype=torch.float32, device=dev)
+    _check(lib().yume_pdq_amd_dihedrals_batch(
+        C.c_void_p(pdqf.data_ptr()), C.c_void_p(thr_in.data_ptr()), n, dihedral_mode,
+        C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+        C.c_void_p(threshold.data_ptr()), C.c_void_p(pds.data_ptr()), _stream_ptr(stream)),
+        "dihedrals_batch")
+    return hashes, quality, threshold, pds
+
+
+HIT_DTYPE = np.dtype([("query", "<u4"), ("distance", "<u4"), ("index", "<u8")])
+
+
+def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, hits=None,
+          count=None, append=False, sync=True):
+    """All (query, index, distance) with popcount(query ^ row) <= threshold.
+
+    db: CUDA u8 tensor (n_rows, 32); queries: CUDA u8 tensor (n_queries, 32).
+    Returns (total_count, hits ndarray of HIT_DTYPE sorted by (index, query))
+    when sync, else the device (hits, count) tensors."""
+    torch = _torch()
+    db = db.reshape(-1, 32)
+    queries = queries.reshape(-1, 32)
+    if not (db.is_cuda and queries.is_cuda) or db.dtype != torch.uint8 or queries.dtype != torch.uint8:
+        raise ValueError("db and queries must be CUDA uint8 tensors of 32-byte rows")
+    db = db.contiguous()
+    queries = queries.contiguous()
+    dev = db.device
+    if hits is None:
+        hits = torch.empty((max(capacity, 1), 16), dtype=torch.uint8, device=dev)
+    if count is None:
+        count = torch.zeros(1, dtype=torch.int64, device=dev)
+    capacity = hits.shape[0]
+    _check(lib().yume_pdq_amd_match(
+        C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
+        int(threshold), C.c_void_p(hits.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
+        index_base, MATCH_APPEND if append else 0, _stream_ptr(stream)), "match")
+    if not sync:
+        return hits, count
+    total = int(count.item())
+    k = min(total, capacity)
+    arr = hits[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
+    arr = np.sort(arr, order=("index", "query"))
+    return total, arr
+
+
+def match_first(db, queries, threshold, stream=None):
+    """(row, query) of the first hit in (row, query) order, or None."""
+    torch = _torch()
+    db = db.reshape(-1, 32).contiguous()
+    queries = queries.reshape(-1, 32).contiguous()
+    first = torch.empty(1, dtype=torch.int64, device=db.device)
+    _check(lib().yume_pdq_amd_match_first(
+        C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
+        int(threshold), C.c_void_p(first.data_ptr()), _stream_ptr(stream)), "match_first")
+    key = int(first.item()) & 0xFFFFFFFFFFFFFFFF
+    if key == 0xFFFFFFFFFFFFFFFF:
+        return None
+    nq = queries.shape[0]
+    return key // nq, key % nq
