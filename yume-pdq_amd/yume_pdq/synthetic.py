@@ -79,3 +79,53 @@ def brute_force_hits(db_u8, queries_u8, threshold, chunk=1 << 20):
     t = np.concatenate(out).astype(np.int64)
     order = np.lexsort((t[:, 0], t[:, 1]))
     return t[order]
+
+
+def image_params(k):
+    """The random parameters image_u8(k) draws, in the same order."""
+    rng = np.random.Generator(np.random.PCG64(IMAGE_SEED_BASE + k))
+    c0 = rng.uniform(40.0, 215.0)
+    cx, cy = rng.uniform(-0.2, 0.2, size=2)
+    blobs = []
+    for _ in range(int(rng.integers(4, 17))):
+        bx, by = rng.uniform(0.0, 512.0, size=2)
+        amp = rng.uniform(20.0, 120.0) * (1.0 if rng.random() < 0.5 else -1.0)
+        sig = rng.uniform(10.0, 80.0)
+        blobs.append((bx, by, amp, sig))
+    return c0, cx, cy, blobs
+
+
+def images_u8_device(n, device, start=0, seed=99):
+    """Bulk images of the same family rendered on the GPU (torch), for timing
+    workloads: same parameter draws as image_u8, noise from torch's generator,
+    so pixels are not bit-identical to image_u8."""
+    import torch
+    y, x = torch.meshgrid(torch.arange(512, device=device, dtype=torch.float32),
+                          torch.arange(512, device=device, dtype=torch.float32), indexing="ij")
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    out = torch.empty((n, 512, 512), dtype=torch.uint8, device=device)
+    for i in range(n):
+        c0, cx, cy, blobs = image_params(start + i)
+        img = c0 + cx * x + cy * y
+        for bx, by, amp, sig in blobs:
+            img = img + amp * torch.exp(-((x - bx) ** 2 + (y - by) ** 2) / (2.0 * sig * sig))
+        img = img + (torch.rand((512, 512), generator=gen, device=device) * 16.0 - 8.0)
+        out[i] = torch.clamp(torch.round(img), 0, 255).to(torch.uint8)
+    return out
+
+
+def plant_fast(db_u8, queries_u8, n_plants, seed=11, max_dist=40):
+    """Vectorised plant(): distinct rows, random queries, U{0..max_dist} distinct flips."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = db_u8.shape[0]
+    rows = rng.choice(n, size=n_plants, replace=False)
+    qids = rng.integers(0, queries_u8.shape[0], size=n_plants)
+    dists = rng.integers(0, max_dist + 1, size=n_plants)
+    order = np.argsort(rng.random((n_plants, 256)), axis=1)
+    mask = np.arange(256)[None, :] < dists[:, None]
+    bits = np.zeros((n_plants, 256), np.uint8)
+    np.put_along_axis(bits, order, mask.astype(np.uint8), axis=1)
+    flips = np.packbits(bits, axis=1, bitorder="little")
+    db_u8[rows] = queries_u8[qids] ^ flips
+    return rows, qids, dists
