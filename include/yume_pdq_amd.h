@@ -59,6 +59,8 @@ typedef struct YumePdqAmdHit {
 
 /* flags for yume_pdq_amd_match */
 #define YUME_PDQ_AMD_MATCH_APPEND 1u /* keep *d_count: append after earlier hits */
+#define YUME_PDQ_AMD_MATCH_VALU 2u   /* force the vector-ALU kernel */
+#define YUME_PDQ_AMD_MATCH_MFMA 4u   /* force the matrix-core kernel (threshold < 64) */
 
 const char *yume_pdq_amd_strerror(int code);
 
@@ -100,7 +102,11 @@ int yume_pdq_amd_dihedrals_batch(const float *d_pdqf, const float *d_thr_in, uin
  * d_hits (in no particular order; sort by (index, query) for the reference's
  * visiting order) up to `capacity`; *d_count receives the exact number of
  * hits (reset first unless flags has YUME_PDQ_AMD_MATCH_APPEND).  The call
- * does not wait; read *d_count after synchronising `stream`.
+ * does not wait; read *d_count after synchronising `stream`.  Batches of 32+
+ * queries at threshold < 64 run on the matrix cores (an exact fp4 +-1 dot
+ * product over the first 128 bits as a filter, then the full 256-bit integer
+ * distance for every surviving pair); smaller batches on the vector ALUs.
+ * Both give the same set; flags MATCH_VALU / MATCH_MFMA force one.
  */
 int yume_pdq_amd_match(const uint8_t *d_db, uint64_t n_rows, const uint8_t *d_queries,
                        uint32_t n_queries, uint32_t threshold, YumePdqAmdHit *d_hits,

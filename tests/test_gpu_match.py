@@ -61,20 +61,37 @@ def test_planted_controls(gpu, oracle, dev, nq):
         assert first == (int(pos), i)
 
 
-@pytest.mark.parametrize("threshold", [0, 1, 31, 64, 128, 200, 255, 256])
-def test_thresholds(gpu, oracle, dev, threshold):
+THRESHOLD_CASES = [(t, p) for t in (0, 1, 31, 63, 64, 128, 200, 255, 256) for p in ("valu", "mfma")
+                   if not (p == "mfma" and t >= 64)]
+
+
+@pytest.mark.parametrize("threshold,path", THRESHOLD_CASES)
+def test_thresholds(gpu, oracle, dev, threshold, path):
     import torch
     db = synthetic.random_db(3000, seed=3)
-    q = synthetic.random_db(8, seed=4)
-    rng = np.random.default_rng(9)
-    synthetic.plant(db, q, 200, seed=5, max_dist=60)
-    say(f"[match] threshold={threshold}")
-    exp_n, exp = oracle.match_all(db, q, threshold, cap=3000 * 8)
+    q = synthetic.random_db(40, seed=4)
+    synthetic.plant(db, q, 200, seed=5, max_dist=70)
+    say(f"[match] threshold={threshold} path={path}")
+    exp_n, exp = oracle.match_all(db, q, threshold, cap=3000 * 40)
     total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), threshold,
-                            capacity=3000 * 8)
+                            capacity=3000 * 40, path=path)
     assert total == exp_n
     assert np.array_equal(sort_triples(as_triples(hits)), exp)
-    del rng
+
+
+@pytest.mark.parametrize("nq", [1, 15, 16, 17, 33, 100, 257, 1000])
+def test_mfma_query_counts(gpu, oracle, dev, nq):
+    """Ragged query counts on the matrix-core path (partial 16-query tiles,
+    partial query groups) against the oracle, ragged row counts too."""
+    import torch
+    db = synthetic.random_db(5003, seed=40 + nq)
+    q = synthetic.random_db(nq, seed=41)
+    synthetic.plant(db, q, 300, seed=42, max_dist=45)
+    say(f"[match] mfma nq={nq}")
+    exp_n, exp = oracle.match_all(db, q, 31)
+    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 31, path="mfma")
+    assert total == exp_n
+    assert np.array_equal(sort_triples(as_triples(hits)), exp)
 
 
 def test_empty_and_tiny(gpu, dev):
@@ -133,3 +150,23 @@ def test_scale_exact_set(gpu, dev, n_rows, nq):
     exp = synthetic.brute_force_hits(db, q, 31)
     assert total == exp.shape[0]
     assert np.array_equal(as_triples(hits), exp)
+
+
+def test_scale_10m_x_1024_paths_agree(gpu, dev):
+    """10M rows x 1024 queries (8 x 128 images' worth), 50k plants: the
+    matrix-core and vector-ALU kernels return the identical set, and the
+    first 500k rows agree with a numpy brute force."""
+    import torch
+    db = synthetic.random_db(10_000_000, seed=7)
+    q = synthetic.random_db(1024, seed=8)
+    synthetic.plant_fast(db, q, 50_000, seed=12)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
+    say("[match] 10M x 1024 mfma")
+    n1, h1 = gpu.match(d_db, d_q, 31, capacity=1 << 20, path="mfma")
+    say("[match] 10M x 1024 valu")
+    n2, h2 = gpu.match(d_db, d_q, 31, capacity=1 << 20, path="valu")
+    assert n1 == n2 and np.array_equal(as_triples(h1), as_triples(h2))
+    assert n1 >= 0.9 * 50_000 * 32 / 41  # plants at d <= 31 are hits
+    sub = synthetic.brute_force_hits(db[:500_000], q, 31)
+    t1 = as_triples(h1)
+    assert np.array_equal(t1[t1[:, 1] < 500_000], sub)

@@ -270,8 +270,16 @@ int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_qu
     if (e != hipSuccess) return from_hip(e);
   }
   static_assert(sizeof(YumePdqAmdHit) == sizeof(ypdq::MatchHit), "hit layout");
-  return from_hip(ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold,
-                                     reinterpret_cast<ypdq::MatchHit*>(d_hits), capacity, d_count, index_base, st));
+  ypdq::MatchHit* hits = reinterpret_cast<ypdq::MatchHit*>(d_hits);
+  const bool force_valu = flags & YUME_PDQ_AMD_MATCH_VALU, force_mfma = flags & YUME_PDQ_AMD_MATCH_MFMA;
+  if (force_valu && force_mfma) return YUME_PDQ_AMD_EINVAL;
+  if (force_mfma && threshold >= 64) return YUME_PDQ_AMD_EINVAL;
+  const bool mfma = force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold));
+  if (mfma)
+    return from_hip(ypdq::launch_match_mfma(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count,
+                                            index_base, st));
+  return from_hip(
+      ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count, index_base, st));
 }
 
 int yume_pdq_amd_match_first(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,

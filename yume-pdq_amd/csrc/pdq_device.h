@@ -49,9 +49,26 @@ hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride
                        const HashOutputs& out, hipStream_t stream);
 hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, uint32_t mode,
                             const HashOutputs& out, hipStream_t stream);
+// Shape of a matcher launch: rows per lane per iteration, threads per block,
+// resident blocks per CU for the grid-stride loop, non-temporal row loads.
+struct MatchTuning {
+  int rows_per_lane;
+  int block;
+  int blocks_per_cu;
+  bool nontemporal;
+};
+MatchTuning default_match_tuning(uint64_t n_rows, uint32_t n_queries);
+hipError_t launch_match_tuned(const MatchTuning& t, const void* db, uint64_t n_rows, const void* queries,
+                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                              unsigned long long* counter, uint64_t index_base, hipStream_t stream);
 hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
                         MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
                         hipStream_t stream);
+// Matrix-core prefix filter + exact re-check (threshold < 64, >= 32 queries).
+bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold);
+hipError_t launch_match_mfma(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
+                             uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                             uint64_t index_base, hipStream_t stream);
 hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
                               uint32_t threshold, unsigned long long* first_key, hipStream_t stream);
 

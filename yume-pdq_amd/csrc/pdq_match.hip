@@ -24,8 +24,6 @@
 
 namespace ypdq {
 
-constexpr int kMatchBlock = 256;
-constexpr int kRowsPerLane = 2;
 
 // popcount(x) + acc: one v_bcnt_u32_b32
 __device__ __forceinline__ uint32_t popc(uint32_t x, uint32_t acc) { return (uint32_t)__builtin_popcount(x) + acc; }
@@ -49,25 +47,31 @@ __device__ __forceinline__ uint32_t dist128(const uint4& r, const uint32_t* q) {
   return popc(r.w ^ q[3], d);
 }
 
-template <bool kFirst>
-__global__ void __launch_bounds__(kMatchBlock) pdq_match_kernel(const uint4* __restrict__ db, uint64_t n_rows,
-                                                                const uint32_t* __restrict__ queries,
-                                                                uint32_t n_queries, uint32_t threshold,
-                                                                MatchHit* __restrict__ hits, uint64_t capacity,
-                                                                unsigned long long* __restrict__ counter,
-                                                                uint64_t index_base) {
-  const uint64_t stride = (uint64_t)gridDim.x * kMatchBlock * kRowsPerLane;
-  for (uint64_t base = ((uint64_t)blockIdx.x * kMatchBlock * kRowsPerLane) + threadIdx.x; base < n_rows;
-       base += stride) {
-    Row256 row[kRowsPerLane];
-    bool valid[kRowsPerLane];
+template <bool kNT>
+__device__ __forceinline__ uint4 load_row_half(const uint4* p) {
+  if (kNT) return load_nt(p);
+  return *p;
+}
+
+// kRows rows per lane per iteration, kBlock threads per block, grid-stride
+// over blocks of kBlock * kRows rows.
+template <bool kFirst, int kRows, bool kNT, int kBlock>
+__global__ void __launch_bounds__(kBlock) pdq_match_kernel(const uint4* __restrict__ db, uint64_t n_rows,
+                                                           const uint32_t* __restrict__ queries, uint32_t n_queries,
+                                                           uint32_t threshold, MatchHit* __restrict__ hits,
+                                                           uint64_t capacity, unsigned long long* __restrict__ counter,
+                                                           uint64_t index_base) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock * kRows;
+  for (uint64_t base = ((uint64_t)blockIdx.x * kBlock * kRows) + threadIdx.x; base < n_rows; base += stride) {
+    Row256 row[kRows];
+    bool valid[kRows];
 #pragma unroll
-    for (int u = 0; u < kRowsPerLane; u++) {
-      const uint64_t r = base + (uint64_t)u * kMatchBlock;
+    for (int u = 0; u < kRows; u++) {
+      const uint64_t r = base + (uint64_t)u * kBlock;
       valid[u] = r < n_rows;
       if (valid[u]) {
-        row[u].lo = load_nt(db + 2 * r);
-        row[u].hi = load_nt(db + 2 * r + 1);
+        row[u].lo = load_row_half<kNT>(db + 2 * r);
+        row[u].hi = load_row_half<kNT>(db + 2 * r + 1);
       } else {
         row[u].lo = make_uint4(0, 0, 0, 0);
         row[u].hi = make_uint4(0, 0, 0, 0);
@@ -76,12 +80,12 @@ __global__ void __launch_bounds__(kMatchBlock) pdq_match_kernel(const uint4* __r
     for (uint32_t qi = 0; qi < n_queries; qi++) {
       const uint32_t* q = queries + 8 * qi;
 #pragma unroll
-      for (int u = 0; u < kRowsPerLane; u++) {
+      for (int u = 0; u < kRows; u++) {
         const uint32_t d0 = dist128(row[u].lo, q);
         if (__builtin_expect(d0 <= threshold && valid[u], 0)) {
           const uint32_t d = d0 + dist128(row[u].hi, q + 4);
           if (d <= threshold) {
-            const uint64_t r = base + (uint64_t)u * kMatchBlock;
+            const uint64_t r = base + (uint64_t)u * kBlock;
             if (kFirst) {
               atomicMin(counter, (unsigned long long)(r * n_queries + qi));
             } else {
@@ -101,36 +105,80 @@ __global__ void __launch_bounds__(kMatchBlock) pdq_match_kernel(const uint4* __r
   }
 }
 
-static int match_grid(uint64_t n_rows) {
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
+static int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
     hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) ? p.multiProcessorCount
+                                                                                             : 256;
   }
-  const uint64_t per_block = (uint64_t)kMatchBlock * kRowsPerLane;
-  uint64_t need = (n_rows + per_block - 1) / per_block;
-  uint64_t cap = (uint64_t)cus * 16;  // enough waves in flight to cover HBM latency
-  if (need > cap) need = cap;
-  return (int)(need ? need : 1);
+  return cus;
 }
 
-hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
-                        MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
-                        hipStream_t stream) {
-  if (n_rows == 0 || n_queries == 0) return hipSuccess;
-  hipLaunchKernelGGL(pdq_match_kernel<false>, dim3(match_grid(n_rows)), dim3(kMatchBlock), 0, stream,
+template <bool kFirst, int kRows, bool kNT, int kBlock>
+static hipError_t launch_cfg(const MatchTuning& t, const void* db, uint64_t n_rows, const void* queries,
+                             uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                             unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
+  const uint64_t per_block = (uint64_t)kBlock * kRows;
+  uint64_t need = (n_rows + per_block - 1) / per_block;
+  const uint64_t cap = (uint64_t)device_cus() * (t.blocks_per_cu > 0 ? t.blocks_per_cu : 8);
+  if (need > cap) need = cap;
+  if (need == 0) need = 1;
+  hipLaunchKernelGGL((pdq_match_kernel<kFirst, kRows, kNT, kBlock>), dim3((unsigned)need), dim3(kBlock), 0, stream,
                      static_cast<const uint4*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
                      threshold, hits, capacity, counter, index_base);
   return hipGetLastError();
 }
 
+template <bool kFirst>
+static hipError_t dispatch(const MatchTuning& t, const void* db, uint64_t n_rows, const void* queries,
+                           uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                           unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
+#define YPDQ_CASE(R, NT, B)                                                                                  \
+  if (t.rows_per_lane == R && t.nontemporal == NT && t.block == B)                                         \
+    return launch_cfg<kFirst, R, NT, B>(t, db, n_rows, queries, n_queries, threshold, hits, capacity, counter, \
+                                        index_base, stream);
+  YPDQ_CASE(1, true, 256) YPDQ_CASE(2, true, 256) YPDQ_CASE(4, true, 256) YPDQ_CASE(8, true, 256)
+  YPDQ_CASE(1, false, 256) YPDQ_CASE(2, false, 256) YPDQ_CASE(4, false, 256) YPDQ_CASE(8, false, 256)
+  YPDQ_CASE(2, true, 512) YPDQ_CASE(4, true, 512) YPDQ_CASE(2, false, 512) YPDQ_CASE(4, false, 512)
+  YPDQ_CASE(2, true, 1024) YPDQ_CASE(4, true, 1024)
+#undef YPDQ_CASE
+  return hipErrorInvalidValue;
+}
+
+MatchTuning default_match_tuning(uint64_t n_rows, uint32_t n_queries) {
+  (void)n_rows;
+  (void)n_queries;
+  MatchTuning t;
+  // measured on MI355X, 10M rows x 8 queries: 512-thread blocks, two rows per
+  // lane, 32 resident blocks per CU, ordinary (cached) loads
+  t.rows_per_lane = 2;
+  t.block = 512;
+  t.blocks_per_cu = 32;
+  t.nontemporal = false;
+  return t;
+}
+
+hipError_t launch_match_tuned(const MatchTuning& t, const void* db, uint64_t n_rows, const void* queries,
+                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                              unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
+  if (n_rows == 0 || n_queries == 0) return hipSuccess;
+  return dispatch<false>(t, db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+}
+
+hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
+                        MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
+                        hipStream_t stream) {
+  return launch_match_tuned(default_match_tuning(n_rows, n_queries), db, n_rows, queries, n_queries, threshold, hits,
+                            capacity, counter, index_base, stream);
+}
+
 hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
                               uint32_t threshold, unsigned long long* first_key, hipStream_t stream) {
   if (n_rows == 0 || n_queries == 0) return hipSuccess;
-  hipLaunchKernelGGL(pdq_match_kernel<true>, dim3(match_grid(n_rows)), dim3(kMatchBlock), 0, stream,
-                     static_cast<const uint4*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
-                     threshold, nullptr, 0, first_key, 0);
-  return hipGetLastError();
+  return dispatch<true>(default_match_tuning(n_rows, n_queries), db, n_rows, queries, n_queries, threshold, nullptr,
+                        0, first_key, 0, stream);
 }
 
 }  // namespace ypdq

@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get("LIBYUME_PDQ_PATH", os.path.join(PKG_DIR, "libyume_pdq
 INPUT_F32, INPUT_U8 = 0, 1
 DIHEDRAL_EXACT, DIHEDRAL_AVX2 = 0, 1
 MATCH_APPEND = 1
+MATCH_VALU = 2
+MATCH_MFMA = 4
 
 # Orientation order of the 8 hashes per image (YUME_PDQ_AMD_D_*).
 DIHEDRALS = ["NORMAL", "FLOPPED", "ROTATED_180", "FLIPPED", "ROTATED_90_FLOPPED",
@@ -207,7 +209,7 @@ HIT_DTYPE = np.dtype([("query", "<u4"), ("distance", "<u4"), ("index", "<u8")])
 
 
 def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, hits=None,
-          count=None, append=False, sync=True):
+          count=None, append=False, sync=True, path=None):
     """All (query, index, distance) with popcount(query ^ row) <= threshold.
 
     db: CUDA u8 tensor (n_rows, 32); queries: CUDA u8 tensor (n_queries, 32).
@@ -229,7 +231,8 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
     _check(lib().yume_pdq_amd_match(
         C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
         int(threshold), C.c_void_p(hits.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
-        index_base, MATCH_APPEND if append else 0, _stream_ptr(stream)), "match")
+        index_base, (MATCH_APPEND if append else 0) | {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path],
+        _stream_ptr(stream)), "match")
     if not sync:
         return hits, count
     total = int(count.item())
