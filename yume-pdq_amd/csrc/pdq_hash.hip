@@ -479,29 +479,43 @@ __device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mod
 // ---------------------------------------------------------------------------
 // Compress + stage 1 over one plane.
 // ---------------------------------------------------------------------------
-// One compressed row i: rows 4i+3 .. 4i+6 arrive from pre[] (the next four
-// are requested first), the chains of row i complete and those of row i+1
-// start, and the two compressed values go into stage-1 accumulator P.
-template <int P, typename T, typename Raw>
+// Rows in flight per wave: D compressed-row steps of four input rows each.
+// u8 rows cost two registers per lane, so the u8 path keeps four steps in
+// flight; f32 rows cost eight and keep one.
+template <typename T>
+struct Prefetch;
+template <>
+struct Prefetch<float> {
+  static constexpr int D = 1;
+};
+template <>
+struct Prefetch<uint8_t> {
+  static constexpr int D = 4;
+};
+
+// One compressed row i (ring slot S = i mod D): rows 4i+3 .. 4i+6 arrive from
+// pre[S], each slot is refilled with the row D steps ahead as soon as it has
+// been converted; the chains of row i complete and those of row i+1 start, and
+// the two compressed values go into stage-1 accumulator P.
+template <int P, int S, int D, typename T, typename Raw>
 __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lane, int i, float (&acc)[16][2][2],
-                                            Chains& cur, Chains& nxt, Raw (&pre)[4], float* __restrict__ buf1) {
-  // rolling prefetch: each row slot is refilled with the row four ahead as
-  // soon as it has been converted
-  const bool more = i + 1 < 127;
+                                            Chains& cur, Chains& nxt, Raw (&pre)[D][4], float* __restrict__ buf1) {
+  const bool more = i + D < 127;
+  const int r0 = 4 * (i + D) + 3;
   Row8 x;
-  to_f32(pre[0], x);
-  if (more) load_raw(plane, 4 * i + 7, lane, pre[0]);
+  to_f32(pre[S][0], x);
+  if (more) load_raw(plane, r0, lane, pre[S][0]);
   chains_feed<2>(cur, x);
-  to_f32(pre[1], x);
-  if (more) load_raw(plane, 4 * i + 8, lane, pre[1]);
+  to_f32(pre[S][1], x);
+  if (more) load_raw(plane, r0 + 1, lane, pre[S][1]);
   chains_feed<3>(cur, x);
   chains_zero(nxt);
-  to_f32(pre[2], x);
-  if (more) load_raw(plane, 4 * i + 9, lane, pre[2]);
+  to_f32(pre[S][2], x);
+  if (more) load_raw(plane, r0 + 2, lane, pre[S][2]);
   chains_feed<4>(cur, x);
   chains_feed<0>(nxt, x);
-  to_f32(pre[3], x);
-  if (more) load_raw(plane, 4 * i + 10, lane, pre[3]);
+  to_f32(pre[S][3], x);
+  if (more) load_raw(plane, r0 + 3, lane, pre[S][3]);
   chains_feed<5>(cur, x);
   chains_feed<1>(nxt, x);
   float c0, c1;
@@ -518,6 +532,8 @@ template <typename T>
 __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int lane, float (&acc)[16][2][2],
                                              float* __restrict__ buf1) {
   using Raw = typename RawRow<T>::type;
+  constexpr int D = Prefetch<T>::D;
+  static_assert(4 % D == 0, "ring depth must divide the unroll factor");
 #pragma unroll
   for (int k = 0; k < 16; k++)
 #pragma unroll
@@ -525,13 +541,15 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
 
   Chains cur, nxt;
   chains_zero(cur);
-  Raw pre[4];
+  Raw pre[D][4];
   {
     Raw r1, r2;
     load_raw(plane, 1, lane, r1);
     load_raw(plane, 2, lane, r2);
 #pragma unroll
-    for (int d = 0; d < 4; d++) load_raw(plane, 3 + d, lane, pre[d]);
+    for (int s = 0; s < D; s++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) load_raw(plane, 4 * s + 3 + d, lane, pre[s][d]);
     Row8 x;
     to_f32(r1, x);
     chains_feed<0>(cur, x);
@@ -539,17 +557,18 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
     chains_feed<1>(cur, x);
   }
   // The accumulator a compressed row feeds is fixed by i mod 4 (alt), so the
-  // loop is unrolled by four to keep every accumulator index a constant.
+  // loop is unrolled by four to keep every accumulator and ring index a
+  // constant.
 #pragma unroll 1
   for (int i = 0; i < 124; i += 4) {
-    stream_step<1>(plane, lane, i, acc, cur, nxt, pre, buf1);
-    stream_step<0>(plane, lane, i + 1, acc, cur, nxt, pre, buf1);
-    stream_step<0>(plane, lane, i + 2, acc, cur, nxt, pre, buf1);
-    stream_step<1>(plane, lane, i + 3, acc, cur, nxt, pre, buf1);
+    stream_step<1, 0 % D, D>(plane, lane, i, acc, cur, nxt, pre, buf1);
+    stream_step<0, 1 % D, D>(plane, lane, i + 1, acc, cur, nxt, pre, buf1);
+    stream_step<0, 2 % D, D>(plane, lane, i + 2, acc, cur, nxt, pre, buf1);
+    stream_step<1, 3 % D, D>(plane, lane, i + 3, acc, cur, nxt, pre, buf1);
   }
-  stream_step<1>(plane, lane, 124, acc, cur, nxt, pre, buf1);
-  stream_step<0>(plane, lane, 125, acc, cur, nxt, pre, buf1);
-  stream_step<0>(plane, lane, 126, acc, cur, nxt, pre, buf1);
+  stream_step<1, 124 % D, D>(plane, lane, 124, acc, cur, nxt, pre, buf1);
+  stream_step<0, 125 % D, D>(plane, lane, 125, acc, cur, nxt, pre, buf1);
+  stream_step<0, 126 % D, D>(plane, lane, 126, acc, cur, nxt, pre, buf1);
 }
 
 // Stage-1 rows into LDS: tmp[k][j] = nan0(acc0) + nan0(acc1) (x86.rs:207-224);

@@ -103,13 +103,21 @@ __device__ __forceinline__ void tile_check(const Expanded (&qa)[QT], const Expan
                                            const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
                                            uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
                                            unsigned long long* __restrict__ counter, uint64_t index_base) {
-  float mx = -1.0e30f;
+  // The products are integers in [-128, 128] and the bound is positive, so
+  // comparing float bit patterns as signed integers orders them exactly
+  // (negative floats are negative integers); integer max3 needs no NaN
+  // canonicalisation.  (Element-wise __float_as_int: a bit cast of a vector
+  // element expression is miscompiled to element 0 by this toolchain.)
+  int mx = INT32_MIN;
 #pragma unroll
   for (int t = 0; t < QT; t++) {
     const v4f c = mfma_prefix(qa[t], rb);
-    mx = __builtin_fmaxf(mx, __builtin_fmaxf(__builtin_fmaxf(c[0], c[1]), __builtin_fmaxf(c[2], c[3])));
+    const int c0 = __float_as_int(c.x), c1 = __float_as_int(c.y);
+    const int c2 = __float_as_int(c.z), c3 = __float_as_int(c.w);
+    mx = max(max(mx, c0), c1);
+    mx = max(max(mx, c2), c3);
   }
-  if (__builtin_expect(wave_any(mx >= bound), 0)) {
+  if (__builtin_expect(wave_any(mx >= __float_as_int(bound)), 0)) {
     for (int t = 0; t < QT; t++) {
       const v4f c = mfma_prefix(qa[t], rb);
       float cv[4] = {c[0], c[1], c[2], c[3]};
@@ -119,17 +127,61 @@ __device__ __forceinline__ void tile_check(const Expanded (&qa)[QT], const Expan
   }
 }
 
-// Grid: x = row chunks of kMfmaWaves * kMfmaTilesPerWave * 16 rows, y = query
-// groups of QT * 16 queries.  Each wave holds its query group's operands in
-// registers and streams its own 64 row tiles.
+template <int QT, bool kFull>
+__device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uint32_t* __restrict__ lut,
+                                             const uint32_t* __restrict__ db, uint64_t row0, uint64_t n_rows,
+                                             uint32_t qbase, int rn, int kq, float bound,
+                                             const uint32_t* __restrict__ queries, uint32_t n_queries,
+                                             uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
+                                             unsigned long long* __restrict__ counter, uint64_t index_base) {
+  constexpr int kAhead = 4;
+  // this lane's word of row (row0 + off): db_lane[off * 8]
+  const uint32_t* __restrict__ db_lane = db + (row0 + rn) * 8 + kq;
+  const uint32_t left = kFull ? 0u : (uint32_t)(n_rows - row0);  // rows of this wave that exist
+  auto fetch = [&](uint32_t tile) -> uint32_t {
+    const uint32_t off = tile * 16;
+    if (kFull) return db_lane[(size_t)off * 8];
+    return (off + rn < left) ? db_lane[(size_t)off * 8] : 0u;
+  };
+  uint32_t ring[kAhead];
+#pragma unroll
+  for (int a = 0; a < kAhead; a++) ring[a] = fetch(a);
+  Expanded cur;
+  expand_word(ring[0], lut, cur);
+  const uint32_t tiles = kFull ? (uint32_t)kMfmaTilesPerWave : (left + 15) / 16;
+#pragma unroll 1
+  for (uint32_t tile = 0; tile < tiles; tile += kAhead) {
+#pragma unroll
+    for (int a = 0; a < kAhead; a++) {
+      const uint32_t t = tile + a;
+      ring[a] = (t + kAhead < tiles) ? fetch(t + kAhead) : 0u;
+      if (!kFull && t >= tiles) break;
+      Expanded rb = cur;
+      if (!kFull && t * 16 + rn >= left) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
+      expand_word(ring[(a + 1) % kAhead], lut, cur);  // operand of tile t + 1
+      tile_check<QT>(qa, rb, row0 + (uint64_t)t * 16, qbase, rn, kq, bound, queries, db, n_queries, n_rows,
+                     threshold, hits, capacity, counter, index_base);
+    }
+  }
+}
+
+// Grid: one dimension of row_chunks x query_groups blocks (row chunks of
+// kMfmaWaves * kMfmaTilesPerWave * 16 rows, query groups of QT * 16 queries).
+// Consecutive workgroups are dealt round-robin to the 8 XCDs, each with its
+// own L2; block b is mapped so that all query groups of a row chunk run on
+// the same XCD (row chunk = xcd + 8 * (slot / groups)) and read its rows from
+// that XCD's L2 after the first one brought them in from HBM.
 template <int QT>
-__global__ void __launch_bounds__(64 * kMfmaWaves) pdq_match_mfma_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, uint64_t n_rows, const uint32_t* __restrict__ queries, uint32_t n_queries,
     uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter,
-    uint64_t index_base) {
+    uint64_t index_base, uint32_t groups) {
   __shared__ uint32_t lut[256];
   lut[threadIdx.x] = spread_byte(threadIdx.x);  // blockDim.x == 256
   __syncthreads();
+  const uint32_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const uint64_t chunk = xcd + 8ull * (slot / groups);
+  const uint32_t group = slot % groups;
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -138,7 +190,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) pdq_match_mfma_kernel(
   const float bound = (float)(128 - 2 * (int)threshold);
 
   // query operands: tile t, query qbase + 16t + rn, word kq
-  const uint32_t qbase = blockIdx.y * (uint32_t)(QT * 16);
+  const uint32_t qbase = group * (uint32_t)(QT * 16);
   Expanded qa[QT];
 #pragma unroll
   for (int t = 0; t < QT; t++) {
@@ -151,35 +203,18 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) pdq_match_mfma_kernel(
   }
 
   const uint64_t wave_rows = (uint64_t)kMfmaTilesPerWave * 16;
-  const uint64_t row0 = ((uint64_t)blockIdx.x * kMfmaWaves + wave) * wave_rows;
+  const uint64_t row0 = (chunk * kMfmaWaves + wave) * wave_rows;
   if (row0 >= n_rows) return;
 
-  // four tiles in flight per wave: one 32-bit word per lane per tile
-  constexpr int kAhead = 4;
-  uint32_t ring[kAhead];
-#pragma unroll
-  for (int a = 0; a < kAhead; a++) {
-    const uint64_t r = row0 + 16 * a + rn;
-    ring[a] = r < n_rows ? db[r * 8 + kq] : 0u;
-  }
-#pragma unroll 1
-  for (int tile = 0; tile < kMfmaTilesPerWave && row0 + (uint64_t)tile * 16 < n_rows; tile += kAhead) {
-#pragma unroll
-    for (int a = 0; a < kAhead; a++) {
-      const uint64_t tbase = row0 + (uint64_t)(tile + a) * 16;
-      const uint32_t w = ring[a];
-      {
-        const uint64_t r = tbase + 16 * kAhead + rn;
-        ring[a] = (tile + a + kAhead < kMfmaTilesPerWave && r < n_rows) ? db[r * 8 + kq] : 0u;
-      }
-      if (tbase >= n_rows) break;
-      Expanded rb;
-      expand_word(w, lut, rb);
-      if (tbase + rn >= n_rows) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
-      tile_check<QT>(qa, rb, tbase, qbase, rn, kq, bound, queries, db, n_queries, n_rows, threshold, hits, capacity,
-                     counter, index_base);
-    }
-  }
+  // Rows stream in four tiles ahead (one 32-bit word per lane per tile); the
+  // next tile's operand is expanded while the current tile's products run.
+  // Only the wave that reaches the end of the database needs bounds checks.
+  if (row0 + wave_rows <= n_rows)
+    stream_tiles<QT, true>(qa, lut, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries, threshold, hits,
+                           capacity, counter, index_base);
+  else
+    stream_tiles<QT, false>(qa, lut, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries, threshold, hits,
+                            capacity, counter, index_base);
 }
 
 template <int QT>
@@ -187,27 +222,41 @@ static hipError_t launch_qt(const void* db, uint64_t n_rows, const void* queries
                             uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
                             uint64_t index_base, hipStream_t stream) {
   const uint64_t rows_per_block = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
-  const uint64_t gx = (n_rows + rows_per_block - 1) / rows_per_block;
-  const uint32_t gy = (n_queries + QT * 16 - 1) / (QT * 16);
-  if (gx > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((pdq_match_mfma_kernel<QT>), dim3((unsigned)gx, gy), dim3(64 * kMfmaWaves), 0, stream,
+  const uint64_t chunks = (n_rows + rows_per_block - 1) / rows_per_block;
+  const uint64_t chunks8 = (chunks + 7) / 8 * 8;  // whole rounds over the 8 XCDs
+  const uint32_t groups = (n_queries + QT * 16 - 1) / (QT * 16);
+  const uint64_t blocks = chunks8 * groups;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((pdq_match_mfma_kernel<QT>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
                      static_cast<const uint32_t*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
-                     threshold, hits, capacity, counter, index_base);
+                     threshold, hits, capacity, counter, index_base, groups);
   return hipGetLastError();
 }
 
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold) { return threshold < 64 && n_queries >= 32; }
 
-hipError_t launch_match_mfma(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
-                             uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                             uint64_t index_base, hipStream_t stream) {
+hipError_t launch_match_mfma_qt(int max_qt, const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
+                                uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                                uint64_t index_base, hipStream_t stream) {
   if (n_rows == 0 || n_queries == 0) return hipSuccess;
   if (threshold >= 64) return hipErrorInvalidValue;
   const uint32_t tiles = (n_queries + 15) / 16;
-  if (tiles <= 2) return launch_qt<2>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
-  if (tiles <= 4) return launch_qt<4>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
-  if (tiles <= 8) return launch_qt<8>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+  if (tiles <= 2 || max_qt == 2)
+    return launch_qt<2>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+  if (tiles <= 4 || max_qt == 4)
+    return launch_qt<4>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+  if (tiles <= 8 || max_qt == 8)
+    return launch_qt<8>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
   return launch_qt<16>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+}
+
+hipError_t launch_match_mfma(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
+                             uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                             uint64_t index_base, hipStream_t stream) {
+  // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
+  // 1024 / 8192 queries); larger tiles cost occupancy
+  return launch_match_mfma_qt(8, db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
+                              stream);
 }
 
 }  // namespace ypdq
