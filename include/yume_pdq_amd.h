@@ -123,6 +123,47 @@ int yume_pdq_amd_match_first(const uint8_t *d_db, uint64_t n_rows, const uint8_t
                              uint32_t n_queries, uint32_t threshold, unsigned long long *d_first,
                              void *stream);
 
+/*
+ * Staged entry points: one GPU launch per method of the reference's Kernel
+ * trait (src/kernel/mod.rs:268-497), with the AVX2 smart kernel's arithmetic.
+ * n images per call; every buffer is device memory laid out image after image.
+ */
+/* Kernel::jarosz_compress (x86.rs:626-682): planes -> n x 128 x 128 (row and
+ * column 127 zero). */
+int yume_pdq_amd_stage_compress(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
+                                uint32_t n, float *d_buf1, void *stream);
+/* Kernel::dct2d (x86.rs:100-281): n x 128 x 128 -> n x 16 x 16; d_tmp (n x 128,
+ * nullable) receives the last stage-1 row as the reference leaves it. */
+int yume_pdq_amd_stage_dct2d(const float *d_buf1, uint32_t n, float *d_tmp, float *d_pdqf, void *stream);
+/* Kernel::sum_of_gradients (x86.rs:840-865): n PDQFs -> n floats. */
+int yume_pdq_amd_stage_sum_of_gradients(const float *d_pdqf, uint32_t n, float *d_out, void *stream);
+/* Kernel::quantize (x86.rs:440-621): n PDQFs -> n thresholds, n x 32 hash bytes. */
+int yume_pdq_amd_stage_quantize(const float *d_pdqf, uint32_t n, float *d_threshold, uint8_t *d_hash,
+                                void *stream);
+/* threshold_2d_f32 (threshold.rs:37-119): n PDQFs + n thresholds -> n x 32. */
+int yume_pdq_amd_stage_threshold(const float *d_pdqf, const float *d_threshold, uint32_t n,
+                                 uint8_t *d_hash, void *stream);
+
+/* In-place PDQF transforms (Kernel::pdqf_t, pdqf_negate_alt_cols/rows,
+ * pdqf_negate_off_diagonals; mod.rs:345-405, x86.rs:285-436). */
+enum {
+  YUME_PDQ_AMD_T_TRANSPOSE = 0,
+  YUME_PDQ_AMD_T_NEG_COLS_EVEN_AVX2 = 1, /* negate_alt_cols::<false>, AVX2 extent (cols 0..7) */
+  YUME_PDQ_AMD_T_NEG_COLS_ODD_AVX2 = 2,  /* negate_alt_cols::<true>,  AVX2 extent */
+  YUME_PDQ_AMD_T_NEG_COLS_EVEN = 3,      /* negate_alt_cols::<false>, all columns */
+  YUME_PDQ_AMD_T_NEG_COLS_ODD = 4,       /* negate_alt_cols::<true>,  all columns */
+  YUME_PDQ_AMD_T_NEG_ROWS_ODD = 5,       /* negate_alt_rows::<false> (rows 1,3,..) */
+  YUME_PDQ_AMD_T_NEG_ROWS_EVEN = 6,      /* negate_alt_rows::<true>  (rows 0,2,..) */
+  YUME_PDQ_AMD_T_NEG_OFF_DIAGONALS = 7
+};
+int yume_pdq_amd_stage_transform(float *d_pdqf, uint32_t n, int op, void *stream);
+
+/* Kernel::cvt_rgb8_to_luma8f / cvt_rgba8_to_luma8f (mod.rs:303-340,
+ * x86.rs:686-753): n_pixels of 3 or 4 bytes -> n_pixels floats, coefficients
+ * (r, g, b) (BT.601: 0.299, 0.587, 0.114, constants.rs:23). */
+int yume_pdq_amd_stage_luma(const uint8_t *d_pixels, int channels, uint64_t n_pixels, float r_coeff,
+                            float g_coeff, float b_coeff, float *d_out, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

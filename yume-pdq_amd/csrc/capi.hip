@@ -293,4 +293,54 @@ int yume_pdq_amd_match_first(const uint8_t* d_db, uint64_t n_rows, const uint8_t
   return from_hip(ypdq::launch_match_first(d_db, n_rows, d_queries, n_queries, threshold, d_first, st));
 }
 
+int yume_pdq_amd_stage_compress(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n,
+                                float* d_buf1, void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_planes || !d_buf1) return YUME_PDQ_AMD_EINVAL;
+  if (input_kind != YUME_PDQ_AMD_INPUT_F32 && input_kind != YUME_PDQ_AMD_INPUT_U8) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = 512ull * 512 * (input_kind == YUME_PDQ_AMD_INPUT_F32 ? 4 : 1);
+  if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 4) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_compress(d_planes, input_kind, plane_stride_bytes, n, d_buf1,
+                                              static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_dct2d(const float* d_buf1, uint32_t n, float* d_tmp, float* d_pdqf, void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_buf1 || !d_pdqf) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_dct2d(d_buf1, n, d_tmp, d_pdqf, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_sum_of_gradients(const float* d_pdqf, uint32_t n, float* d_out, void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pdqf || !d_out || (uintptr_t)d_pdqf % 16) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_gradient(d_pdqf, n, d_out, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_quantize(const float* d_pdqf, uint32_t n, float* d_threshold, uint8_t* d_hash, void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pdqf || !d_threshold || !d_hash) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_quantize(d_pdqf, n, d_threshold, d_hash, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_threshold(const float* d_pdqf, const float* d_threshold, uint32_t n, uint8_t* d_hash,
+                                 void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pdqf || !d_threshold || !d_hash) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_threshold(d_pdqf, d_threshold, n, d_hash, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_transform(float* d_pdqf, uint32_t n, int op, void* stream) {
+  if (n == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pdqf || op < 0 || op > 7) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_stage_transform(d_pdqf, n, op, static_cast<hipStream_t>(stream)));
+}
+
+int yume_pdq_amd_stage_luma(const uint8_t* d_pixels, int channels, uint64_t n_pixels, float r_coeff, float g_coeff,
+                            float b_coeff, float* d_out, void* stream) {
+  if (n_pixels == 0) return YUME_PDQ_AMD_OK;
+  if (!d_pixels || !d_out || (channels != 3 && channels != 4)) return YUME_PDQ_AMD_EINVAL;
+  const float c[3] = {r_coeff, g_coeff, b_coeff};
+  return from_hip(ypdq::launch_stage_luma(d_pixels, channels, n_pixels, c, d_out, static_cast<hipStream_t>(stream)));
+}
+
 }  // extern "C"

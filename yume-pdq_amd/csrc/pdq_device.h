@@ -72,4 +72,15 @@ hipError_t launch_match_mfma(const void* db, uint64_t n_rows, const void* querie
 hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
                               uint32_t threshold, unsigned long long* first_key, hipStream_t stream);
 
+// Staged kernels (one per Kernel trait method).
+hipError_t launch_stage_compress(const void* planes, int input_kind, uint64_t stride, uint32_t n, float* buf1,
+                                 hipStream_t s);
+hipError_t launch_stage_dct2d(const float* buf1, uint32_t n, float* tmp, float* pdqf, hipStream_t s);
+hipError_t launch_stage_gradient(const float* pdqf, uint32_t n, float* out, hipStream_t s);
+hipError_t launch_stage_quantize(const float* pdqf, uint32_t n, float* thr, uint8_t* hash, hipStream_t s);
+hipError_t launch_stage_threshold(const float* pdqf, const float* thr, uint32_t n, uint8_t* hash, hipStream_t s);
+hipError_t launch_stage_transform(float* pdqf, uint32_t n, int op, hipStream_t s);
+hipError_t launch_stage_luma(const uint8_t* px, int channels, uint64_t n, const float coeff[3], float* out,
+                             hipStream_t s);
+
 }  // namespace ypdq

@@ -68,6 +68,16 @@ def lib():
         L.yume_pdq_amd_match.argtypes = [vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
         L.yume_pdq_amd_match_first.restype = i
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
+        for name, args in (("compress", [vp, i, u64, u32, vp, vp]),
+                           ("dct2d", [vp, u32, vp, vp, vp]),
+                           ("sum_of_gradients", [vp, u32, vp, vp]),
+                           ("quantize", [vp, u32, vp, vp, vp]),
+                           ("threshold", [vp, vp, u32, vp, vp]),
+                           ("transform", [vp, u32, i, vp]),
+                           ("luma", [vp, i, u64, f, f, f, vp, vp])):
+            fn = getattr(L, "yume_pdq_amd_stage_" + name)
+            fn.restype = i
+            fn.argtypes = args
         _lib = L
     return _lib
 
@@ -256,3 +266,105 @@ def match_first(db, queries, threshold, stream=None):
         return None
     nq = queries.shape[0]
     return key // nq, key % nq
+
+
+# ---------------------------------------------------------------------------
+# Staged API: one launch per method of the reference's Kernel trait
+# (src/kernel/mod.rs:268-497), batched over n images of device tensors.
+# ---------------------------------------------------------------------------
+T_TRANSPOSE = 0
+T_NEG_COLS_EVEN_AVX2 = 1
+T_NEG_COLS_ODD_AVX2 = 2
+T_NEG_COLS_EVEN = 3
+T_NEG_COLS_ODD = 4
+T_NEG_ROWS_ODD = 5
+T_NEG_ROWS_EVEN = 6
+T_NEG_OFF_DIAGONALS = 7
+
+# BT.601 luma coefficients (src/kernel/constants.rs:23).
+RGB8_TO_LUMA8_TABLE_ITU = (0.299, 0.587, 0.114)
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr() if t is not None else 0)
+
+
+def stage_compress(planes, stream=None):
+    """Kernel::jarosz_compress on (n,512,512) f32/u8 planes -> (n,128,128) f32."""
+    torch = _torch()
+    if planes.dim() == 2:
+        planes = planes.unsqueeze(0)
+    planes = planes.contiguous()
+    kind, elem = (INPUT_F32, 4) if planes.dtype == torch.float32 else (INPUT_U8, 1)
+    if planes.dtype not in (torch.float32, torch.uint8) or planes.shape[-2:] != (512, 512):
+        raise ValueError("planes must be (n, 512, 512) float32 or uint8")
+    n = planes.shape[0]
+    buf1 = torch.empty((n, 128, 128), dtype=torch.float32, device=planes.device)
+    _check(lib().yume_pdq_amd_stage_compress(_vp(planes), kind, 512 * 512 * elem, n, _vp(buf1),
+                                             _stream_ptr(stream)), "stage_compress")
+    return buf1
+
+
+def stage_dct2d(buf1, stream=None):
+    """Kernel::dct2d on (n,128,128) -> (tmp (n,128), pdqf (n,16,16))."""
+    torch = _torch()
+    buf1 = buf1.reshape(-1, 128, 128).contiguous()
+    n = buf1.shape[0]
+    tmp = torch.empty((n, 128), dtype=torch.float32, device=buf1.device)
+    pdqf = torch.empty((n, 16, 16), dtype=torch.float32, device=buf1.device)
+    _check(lib().yume_pdq_amd_stage_dct2d(_vp(buf1), n, _vp(tmp), _vp(pdqf), _stream_ptr(stream)),
+           "stage_dct2d")
+    return tmp, pdqf
+
+
+def stage_sum_of_gradients(pdqf, stream=None):
+    torch = _torch()
+    pdqf = pdqf.reshape(-1, 16, 16).contiguous()
+    out = torch.empty(pdqf.shape[0], dtype=torch.float32, device=pdqf.device)
+    _check(lib().yume_pdq_amd_stage_sum_of_gradients(_vp(pdqf), pdqf.shape[0], _vp(out),
+                                                     _stream_ptr(stream)), "stage_sum_of_gradients")
+    return out
+
+
+def stage_quantize(pdqf, stream=None):
+    """Kernel::quantize: (n,16,16) -> (threshold (n,), hash (n,32))."""
+    torch = _torch()
+    pdqf = pdqf.reshape(-1, 16, 16).contiguous()
+    n = pdqf.shape[0]
+    thr = torch.empty(n, dtype=torch.float32, device=pdqf.device)
+    h = torch.empty((n, 32), dtype=torch.uint8, device=pdqf.device)
+    _check(lib().yume_pdq_amd_stage_quantize(_vp(pdqf), n, _vp(thr), _vp(h), _stream_ptr(stream)),
+           "stage_quantize")
+    return thr, h
+
+
+def stage_threshold(pdqf, thr, stream=None):
+    """threshold_2d_f32 with one threshold per image: -> hash (n,32)."""
+    torch = _torch()
+    pdqf = pdqf.reshape(-1, 16, 16).contiguous()
+    thr = thr.reshape(-1).contiguous().to(torch.float32)
+    n = pdqf.shape[0]
+    h = torch.empty((n, 32), dtype=torch.uint8, device=pdqf.device)
+    _check(lib().yume_pdq_amd_stage_threshold(_vp(pdqf), _vp(thr), n, _vp(h), _stream_ptr(stream)),
+           "stage_threshold")
+    return h
+
+
+def stage_transform(pdqf, op, stream=None):
+    """In-place PDQF transform (T_* op) of a contiguous (n,16,16) f32 tensor."""
+    if not pdqf.is_contiguous():
+        raise ValueError("stage_transform works in place on a contiguous tensor")
+    _check(lib().yume_pdq_amd_stage_transform(_vp(pdqf), pdqf.numel() // 256, int(op), _stream_ptr(stream)),
+           "stage_transform")
+    return pdqf
+
+
+def stage_luma(pixels, coeff=RGB8_TO_LUMA8_TABLE_ITU, stream=None):
+    """RGB8 / RGBA8 pixels (..., 3 or 4) u8 -> luma f32 (...)."""
+    torch = _torch()
+    ch = pixels.shape[-1]
+    pixels = pixels.contiguous()
+    out = torch.empty(pixels.shape[:-1], dtype=torch.float32, device=pixels.device)
+    _check(lib().yume_pdq_amd_stage_luma(_vp(pixels), ch, out.numel(), float(coeff[0]), float(coeff[1]),
+                                         float(coeff[2]), _vp(out), _stream_ptr(stream)), "stage_luma")
+    return out
