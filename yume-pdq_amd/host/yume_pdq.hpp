@@ -131,7 +131,7 @@ class GpuKernel {
       : neg_(neg),
         d_img_(512 * 512 * sizeof(float)),
         d_buf1_(128 * 128 * sizeof(float)),
-        d_small_(4096) {}
+        d_small_(kSmallBytes) {}
 
   const char* ident() const { return neg_ == ColumnNegation::Avx2 ? "hip_gfx950_f32" : "hip_gfx950_f32_exact"; }
 
@@ -174,16 +174,16 @@ class GpuKernel {
 
   void dct2d(const Buffer1& buffer, RowTmp& tmp, PDQHashF& output) {
     upload(d_buf1_.get(), buffer, sizeof(Buffer1));
-    float* d_tmp = small<float>(0);
-    float* d_p = small<float>(128);
+    float* d_tmp = small<float>(kTmpAt);
+    float* d_p = small<float>(kPdqfAt);
     detail::check(yume_pdq_amd_stage_dct2d(d_buf1_.get<float>(), 1, d_tmp, d_p, stream_.get()), "dct2d");
     download(tmp, d_tmp, sizeof(RowTmp));
     download(output, d_p, sizeof(PDQHashF));
   }
 
   float sum_of_gradients(const PDQHashF& input) {
-    float* d_p = small<float>(0);
-    float* d_g = small<float>(256);
+    float* d_p = small<float>(kPdqfAt);
+    float* d_g = small<float>(kScalarAt);
     upload(d_p, input, sizeof(PDQHashF));
     detail::check(yume_pdq_amd_stage_sum_of_gradients(d_p, 1, d_g, stream_.get()), "sum_of_gradients");
     float g = 0.0f;
@@ -198,9 +198,9 @@ class GpuKernel {
   }
 
   void quantize(const PDQHashF& input, float& threshold, PDQHash& output) {
-    float* d_p = small<float>(0);
-    float* d_t = small<float>(256);
-    uint8_t* d_h = small<uint8_t>(260 * 4);
+    float* d_p = small<float>(kPdqfAt);
+    float* d_t = small<float>(kScalarAt);
+    uint8_t* d_h = small<uint8_t>(kHashAt);
     upload(d_p, input, sizeof(PDQHashF));
     detail::check(yume_pdq_amd_stage_quantize(d_p, 1, d_t, d_h, stream_.get()), "quantize");
     download(&threshold, d_t, sizeof threshold);
@@ -209,9 +209,9 @@ class GpuKernel {
 
   // threshold_2d_f32 (threshold.rs:37-119).
   void threshold_2d(const PDQHashF& input, PDQHash& output, float threshold) {
-    float* d_p = small<float>(0);
-    float* d_t = small<float>(256);
-    uint8_t* d_h = small<uint8_t>(260 * 4);
+    float* d_p = small<float>(kPdqfAt);
+    float* d_t = small<float>(kScalarAt);
+    uint8_t* d_h = small<uint8_t>(kHashAt);
     upload(d_p, input, sizeof(PDQHashF));
     upload(d_t, &threshold, sizeof threshold);
     detail::check(yume_pdq_amd_stage_threshold(d_p, d_t, 1, d_h, stream_.get()), "threshold_2d");
@@ -221,9 +221,11 @@ class GpuKernel {
   hipStream_t stream() const { return stream_.get(); }
 
  private:
+  // Layout of the small device buffer (bytes).
+  static constexpr size_t kPdqfAt = 0, kTmpAt = 1024, kScalarAt = 1536, kHashAt = 1600, kSmallBytes = 2048;
   template <typename T>
-  T* small(size_t offset_elems) const {
-    return reinterpret_cast<T*>(d_small_.get<uint8_t>() + offset_elems * 4);
+  T* small(size_t byte_offset) const {
+    return reinterpret_cast<T*>(d_small_.get<uint8_t>() + byte_offset);
   }
   void upload(void* d, const void* h, size_t n) {
     detail::check_hip(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, stream_.get()), "hipMemcpyAsync");
@@ -233,7 +235,7 @@ class GpuKernel {
     stream_.sync();
   }
   void transform(PDQHashF& p, int op) {
-    float* d_p = small<float>(0);
+    float* d_p = small<float>(kPdqfAt);
     upload(d_p, p, sizeof(PDQHashF));
     detail::check(yume_pdq_amd_stage_transform(d_p, 1, op, stream_.get()), "pdqf transform");
     download(p, d_p, sizeof(PDQHashF));
