@@ -6,13 +6,17 @@ tent compression, DCT, quality, median quantisation, 8 dihedral hashes), then
 match the 8B dihedral hashes against a 10M-row database resident in HBM at
 threshold 31 (exact (query, index, distance) triples into a device buffer),
 then read the hit count back.  The database carries 50,000 planted
-near-duplicates of the batch's queries at distances U{0..40}.
+near-duplicates (U{0..40} distinct bit flips) of the 8192 dihedral hashes of
+1024 images; a batch of B images is the first B of them.
 
-Prints one JSON line (rank 0).  `value` is images/s at the headline batch;
-the other batch sizes, the unbatched f32 path, step latency percentiles, the
-roofline of the dominant kernels (timed with HIP events on the stream they
-run on) and a CPU baseline (reference CPU kernel + matcher restated in C,
-timed on this host) ride along.
+Prints one JSON line.  `value` is images/s at the headline batch (128 images
+= 1024 queries per pass, the largest batch whose step p99 stays under 2 ms);
+the other batch sizes, the plain unbatched f32 path, step latency
+percentiles, the roofline of the step's dominant kernel and of the others
+(each timed as 20-50 back-to-back launches between two HIP events on the
+launch stream), the HBM traffic measured by rocprofv3 counters (read from
+profiles/hbm_traffic_counters.md) and a CPU baseline (the reference's CPU
+kernel + matcher restated in C, timed on this host's cores) ride along.
 """
 import argparse
 import json
@@ -26,6 +30,9 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 
 METRIC = "hash+match images/s @10M DB thr31; p50/p99 latency; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+PLANT_POOL_IMAGES = 1024
+MFMA_FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 matrix peak
+COUNTER_SUMMARY = os.path.join(REPO, "profiles", "hbm_traffic_counters.md")
 
 
 def log(msg):
@@ -48,7 +55,7 @@ def main():
     ap.add_argument("--headline-batch", type=int, default=128)
     ap.add_argument("--threshold", type=int, default=31)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-images", type=int, default=512)
+    ap.add_argument("--cpu-images", type=int, default=8192)
     ap.add_argument("--verify", action="store_true", help="check B=1 hits against numpy (slow)")
     args = ap.parse_args()
 
@@ -71,13 +78,16 @@ def main():
     bmax = max(batches + [args.headline_batch])
     thr = args.threshold
 
-    log(f"[bench] device {info}; generating {bmax} images on the GPU")
-    planes_u8 = synthetic.images_u8_device(bmax, dev)
+    # Plants are drawn from the dihedral hashes of 1024 images whatever batches
+    # run (smaller batches use prefixes), so every run sees the same database.
+    n_imgs = max(bmax, PLANT_POOL_IMAGES)
+    log(f"[bench] device {info}; generating {n_imgs} images on the GPU")
+    planes_u8 = synthetic.images_u8_device(n_imgs, dev)
     planes_f32 = planes_u8[:1].to(torch.float32).contiguous()
     torch.cuda.synchronize()
 
     # queries of the whole image set (used to plant the database)
-    h_all, _, _ = yume_pdq.hash_batch(planes_u8)
+    h_all, _, _ = yume_pdq.hash_batch(planes_u8[:PLANT_POOL_IMAGES])
     queries_all = h_all.reshape(-1, 32).cpu().numpy()
     log(f"[bench] building {args.db_rows} row database with {args.plants} plants")
     db = synthetic.random_db(args.db_rows, seed=7)
@@ -153,39 +163,89 @@ def main():
         assert exp.shape[0] == results.get(1, plain)["hits"], "hit count mismatch"
 
     head = results[args.headline_batch]
-    b1 = results.get(1, plain)
     n_rows = args.db_rows
-    # algorithmic bytes: 32 B per row scanned (+16 B per hit written)
-    match_bytes = 32.0 * n_rows + 16.0 * b1["hits"]
-    match_gbs = match_bytes / (b1["match_kernel_ms"] * 1e-3) / 1e9
-    hb = results[bmax]
-    hash_bytes = 262144.0 * bmax + 320.0 * bmax
-    hash_gbs = hash_bytes / (hb["hash_kernel_ms"] * 1e-3) / 1e9
-    # the headline step's dominant kernel
-    dom_is_match = head["match_kernel_ms"] >= head["hash_kernel_ms"]
+    B = args.headline_batch
 
-    roofline = {
-        "kernel": "pdq_match_kernel: 10M rows x 8 queries (one image's dihedrals), thr 31",
-        "bound": "hbm",
-        "achieved": round(match_gbs, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(match_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "algorithmic_bytes_per_launch": match_bytes,
-        "avg_launch_ms": b1["match_kernel_ms"],
+    # Kernel durations: K back-to-back launches between two HIP events on the
+    # launch stream, mean per launch.
+    def kernel_ms(fn, k=20):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(k):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / k
+
+    q_head = out_bufs[B][0].reshape(-1, 32)
+    q_one = out_bufs[1][0].reshape(-1, 32)
+    log("[bench] back-to-back kernel timing")
+    t_match_head = kernel_ms(lambda: yume_pdq.match(d_db, q_head, thr, hits=d_hits, count=d_count,
+                                                    sync=False, stream=stream))
+    hits_head = int(d_count.item())
+    t_scan8 = kernel_ms(lambda: yume_pdq.match(d_db, q_one, thr, hits=d_hits, count=d_count,
+                                               sync=False, stream=stream), k=50)
+    hits_8 = int(d_count.item())
+    t_hash_head = kernel_ms(lambda: yume_pdq.hash_batch(planes_u8[:B], stream=stream, out=out_bufs[B]))
+    t_hash_max = kernel_ms(lambda: yume_pdq.hash_batch(planes_u8[:bmax], stream=stream, out=out_bufs[bmax]))
+    log(f"[bench]   match {8 * B}q {t_match_head:.4f} ms, scan 8q {t_scan8:.4f} ms, "
+        f"hash B={B} {t_hash_head:.4f} ms, hash B={bmax} {t_hash_max:.4f} ms")
+
+    counters = load_counter_summary()
+    nq_head = 8 * B
+    mfma_path = yume_pdq.match_uses_matrix_cores(nq_head, thr)
+    # Matcher algorithmic bytes: 32 B per database row + 16 B per hit written.
+    bytes_head = 32.0 * n_rows + 16.0 * hits_head
+    bytes_8 = 32.0 * n_rows + 16.0 * hits_8
+    scan8_gbs = bytes_8 / (t_scan8 * 1e-3) / 1e9
+    roofline_scan8 = {
+        "kernel": f"pdq_match_kernel (vector ALU): {n_rows} rows x 8 queries, thr {thr}",
+        "bound": "hbm", "achieved": round(scan8_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(scan8_gbs / HBM_PEAK_GBS, 4),
+        "traffic": counters.get("match_x8", {}).get("traffic_bytes"),
+        "algorithmic_bytes_per_launch": bytes_8, "avg_launch_ms": round(t_scan8, 5),
     }
+    if mfma_path:
+        # 128-bit prefix dot product of +-1 fp4 operands on the matrix cores:
+        # 128 multiply-adds = 256 ops per (row, query) pair.
+        ops = 256.0 * n_rows * nq_head
+        achieved = ops / (t_match_head * 1e-3) / 1e12
+        roofline_match = {
+            "kernel": (f"pdq_match_mfma_kernel (fp4 MFMA 16x16x128 prefix filter + exact 256-bit "
+                       f"recheck): {n_rows} rows x {nq_head} queries, thr {thr}"),
+            "bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_FP4_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / MFMA_FP4_PEAK_TFLOPS, 4),
+            "traffic": counters.get(f"match_x{nq_head}", {}).get("traffic_bytes"),
+            "algorithmic_ops_per_launch": ops, "avg_launch_ms": round(t_match_head, 5),
+            "pairs_per_s": n_rows * nq_head / (t_match_head * 1e-3),
+            "hbm_gbs": round(bytes_head / (t_match_head * 1e-3) / 1e9, 1),
+        }
+    else:
+        gbs = bytes_head / (t_match_head * 1e-3) / 1e9
+        roofline_match = {
+            "kernel": f"pdq_match_kernel (vector ALU): {n_rows} rows x {nq_head} queries, thr {thr}",
+            "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": counters.get(f"match_x{nq_head}", {}).get("traffic_bytes"),
+            "algorithmic_bytes_per_launch": bytes_head, "avg_launch_ms": round(t_match_head, 5),
+        }
+    # Hash algorithmic bytes: the u8 plane (262,144 B) + 8 x (32 + 4 + 4) B out.
+    hash_bytes = (262144.0 + 320.0) * B
+    hash_gbs = hash_bytes / (t_hash_head * 1e-3) / 1e9
     roofline_hash = {
-        "kernel": f"pdq_hash_kernel<u8>: {bmax} images per launch",
-        "bound": "hbm",
-        "achieved": round(hash_gbs, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
+        "kernel": f"pdq_hash_kernel<u8>: {B} images per launch",
+        "bound": "hbm", "achieved": round(hash_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(hash_gbs / HBM_PEAK_GBS, 4),
-        "algorithmic_bytes_per_launch": hash_bytes,
-        "avg_launch_ms": hb["hash_kernel_ms"],
-        "images_per_s_kernel_only": bmax / (hb["hash_kernel_ms"] * 1e-3),
+        "traffic": counters.get(f"hash_u8_b{B}", {}).get("traffic_bytes"),
+        "algorithmic_bytes_per_launch": hash_bytes, "avg_launch_ms": round(t_hash_head, 5),
+        "images_per_s_kernel_only": B / (t_hash_head * 1e-3),
+        f"images_per_s_kernel_only_b{bmax}": bmax / (t_hash_max * 1e-3),
     }
+    dom_is_match = t_match_head >= t_hash_head
+    roofline = roofline_match if dom_is_match else roofline_hash
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -219,13 +279,30 @@ def main():
         "latency_ms": head["latency_ms"],
         "dominant_kernel": "match" if dom_is_match else "hash",
         "roofline": roofline,
+        "roofline_hbm_scan_8_queries": roofline_scan8,
+        "roofline_match": roofline_match,
         "roofline_hash": roofline_hash,
+        "counter_source": counters.get("_source"),
         "batches": [results[b] for b in sorted(results)],
         "plain_unbatched_f32": plain,
         "speedup_vs_plain_unbatched": round(head["images_per_s"] / plain["images_per_s"], 2),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def load_counter_summary():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by the same counters on a copy of known size), from the JSON block
+    of the committed counter summary; {} when absent."""
+    try:
+        text = open(COUNTER_SUMMARY).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        data = json.loads(block)
+        data["_source"] = os.path.relpath(COUNTER_SUMMARY, REPO)
+        return data
+    except (OSError, IndexError, ValueError):
+        return {}
 
 
 def cpu_baseline(args, db, thr, n_rows):
@@ -239,17 +316,18 @@ def cpu_baseline(args, db, thr, n_rows):
     n_img = args.cpu_images
     threads = max(1, min(16, os.cpu_count() or 1))
     log(f"[bench] CPU baseline: {n_img} images, {8 * n_img} queries x {n_rows} rows, {threads} threads")
-    base = synthetic.images_u8(min(n_img, 32)).astype(np.float32)
-    imgs = np.concatenate([base] * ((n_img + base.shape[0] - 1) // base.shape[0]))[:n_img]
+    pool = synthetic.images_u8(min(n_img, 64)).astype(np.float32)
     use512 = bool(oracle_lib.lib().oracle_cpu_has_vpopcntdq())
-    hits, _, t_hash, t_match = oracle_lib.cpu_baseline(imgs, db, thr, threads, use_avx512=use512)
+    hits, _, t_hash, t_match = oracle_lib.cpu_baseline(pool, db, thr, threads, use_avx512=use512,
+                                                       n_images=n_img)
     total = t_hash + t_match
     return {
         "value": round(n_img / total, 2),
         "unit": "images/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{n_img} f32 images hashed (AVX2 kernel restated with intrinsics) and their "
+        "sample": (f"{n_img} f32 images (cycling through {pool.shape[0]} distinct synthetic images) "
+                   f"hashed (AVX2 kernel restated with intrinsics) and their "
                    f"{8 * n_img} dihedral hashes matched against all {n_rows} rows "
                    f"({'AVX-512 VPOPCNTDQ' if use512 else 'scalar u64 popcount'} CpuMatcher), "
                    f"{threads} threads"),

@@ -113,6 +113,12 @@ int yume_pdq_amd_match(const uint8_t *d_db, uint64_t n_rows, const uint8_t *d_qu
                        uint64_t capacity, unsigned long long *d_count, uint64_t index_base,
                        uint32_t flags, void *stream);
 
+/* Which kernel yume_pdq_amd_match would run for these arguments:
+ * YUME_PDQ_AMD_MATCH_VALU or YUME_PDQ_AMD_MATCH_MFMA (or EINVAL for
+ * contradictory flags).  No reference counterpart (path introspection for
+ * schedulers and measurement). */
+int yume_pdq_amd_match_path(uint32_t n_queries, uint32_t threshold, uint32_t flags);
+
 /*
  * First hit in (row, query) order -- what VulkanMatcher::find reports
  * (atomicMin(needle + row * N), src/matching/vulkan.rs:133-136) and where

@@ -44,7 +44,7 @@ float oracle_avx2_hash(const float *in, float *threshold, uint8_t *out, float *b
 int oracle_cpu_has_vpopcntdq(void);
 uint64_t oracle_cpu_match_count(const uint8_t *db, uint64_t n_rows, const uint8_t *queries,
                                 uint32_t n_queries, uint32_t thr, int use_avx512);
-uint64_t oracle_cpu_baseline(const float *images, uint32_t n_images, const uint8_t *db,
+uint64_t oracle_cpu_baseline(const float *images, uint32_t n_pool, uint32_t n_images, const uint8_t *db,
                              uint64_t n_rows, uint32_t thr, uint32_t n_threads, int use_avx512,
                              uint8_t *hashes_out, double *t_hash, double *t_match);
 float oracle_hash_get_threshold(const float *in, float *threshold, uint8_t *out, float *buf1,

@@ -280,12 +280,14 @@ uint64_t oracle_cpu_match_count(const uint8_t *db, uint64_t n_rows, const uint8_
 }
 
 /* ---------------------------------------------------------------------------
- * Threaded CPU baseline: hash n_images (f32 planes) then match the 8
+ * Threaded CPU baseline: hash n_images (f32 planes, cycling through a pool of
+ * n_pool distinct planes) then match the 8
  * dihedral hashes of each image against the DB, with n_threads std-thread
  * style workers (pthreads).  Returns elapsed seconds for each phase.
  * ------------------------------------------------------------------------- */
 typedef struct {
-  const float *images;
+  const float *images; /* pool of n_pool planes; image i is plane i % n_pool */
+  uint32_t n_pool;
   uint32_t first, last;
   uint8_t *hashes; /* 8 x 32 per image */
 } HashJob;
@@ -297,7 +299,7 @@ static void *hash_worker(void *arg) {
   memset(buf1, 0, 128 * 128 * sizeof(float));
   for (uint32_t i = j->first; i < j->last; i++) {
     uint8_t *h = j->hashes + (size_t)i * 256;
-    oracle_avx2_hash(j->images + (size_t)i * 512 * 512, &thr, h, buf1, tmp, pdqf);
+    oracle_avx2_hash(j->images + (size_t)(i % j->n_pool) * 512 * 512, &thr, h, buf1, tmp, pdqf);
     uint32_t codes[7];
     float th[7], q[7], pd[7 * 256];
     oracle_visit_dihedrals(pdqf, thr, 1, codes, th, q, h + 32, pd);
@@ -332,7 +334,7 @@ static double now_s(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-uint64_t oracle_cpu_baseline(const float *images, uint32_t n_images, const uint8_t *db,
+uint64_t oracle_cpu_baseline(const float *images, uint32_t n_pool, uint32_t n_images, const uint8_t *db,
                              uint64_t n_rows, uint32_t thr, uint32_t n_threads, int use_avx512,
                              uint8_t *hashes_out, double *t_hash, double *t_match) {
   init();
@@ -342,6 +344,7 @@ uint64_t oracle_cpu_baseline(const float *images, uint32_t n_images, const uint8
   double t0 = now_s();
   for (uint32_t t = 0; t < n_threads; t++) {
     hj[t].images = images;
+    hj[t].n_pool = n_pool ? n_pool : 1;
     hj[t].first = (uint32_t)((uint64_t)n_images * t / n_threads);
     hj[t].last = (uint32_t)((uint64_t)n_images * (t + 1) / n_threads);
     hj[t].hashes = hashes_out;

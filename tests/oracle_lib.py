@@ -62,7 +62,7 @@ def lib():
             "oracle_match_first": (i, [vp, u64, vp, u32, u32, vp, vp]),
             "oracle_cpu_has_vpopcntdq": (i, []),
             "oracle_cpu_match_count": (u64, [vp, u64, vp, u32, u32, i]),
-            "oracle_cpu_baseline": (u64, [vp, u32, vp, u64, u32, u32, i, vp, vp, vp]),
+            "oracle_cpu_baseline": (u64, [vp, u32, u32, vp, u64, u32, u32, i, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -253,14 +253,16 @@ def cpu_match_count(db, queries, threshold, use_avx512=True):
                                             1 if use_avx512 else 0))
 
 
-def cpu_baseline(images, db, threshold, n_threads, use_avx512=True):
-    """Reference CPU kernel + matcher over images (n,512,512 f32) and db (m,32 u8)."""
+def cpu_baseline(images, db, threshold, n_threads, use_avx512=True, n_images=None):
+    """Reference CPU kernel + matcher over n_images images drawn cyclically from
+    the pool `images` (n,512,512 f32) and db (m,32 u8)."""
     imgs = np.ascontiguousarray(images, np.float32).reshape(-1, 512, 512)
     db = np.ascontiguousarray(db, np.uint8).reshape(-1, 32)
-    hashes = np.zeros((imgs.shape[0], 8, 32), np.uint8)
+    n_images = imgs.shape[0] if n_images is None else int(n_images)
+    hashes = np.zeros((n_images, 8, 32), np.uint8)
     th = C.c_double(0)
     tm = C.c_double(0)
-    hits = lib().oracle_cpu_baseline(_p(imgs), imgs.shape[0], _p(db), db.shape[0], threshold,
+    hits = lib().oracle_cpu_baseline(_p(imgs), imgs.shape[0], n_images, _p(db), db.shape[0], threshold,
                                      n_threads, 1 if use_avx512 else 0, _p(hashes),
                                      C.byref(th), C.byref(tm))
     return int(hits), hashes, th.value, tm.value

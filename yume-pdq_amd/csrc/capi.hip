@@ -282,6 +282,14 @@ int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_qu
       ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count, index_base, st));
 }
 
+int yume_pdq_amd_match_path(uint32_t n_queries, uint32_t threshold, uint32_t flags) {
+  const bool force_valu = flags & YUME_PDQ_AMD_MATCH_VALU, force_mfma = flags & YUME_PDQ_AMD_MATCH_MFMA;
+  if ((force_valu && force_mfma) || (force_mfma && threshold >= 64)) return YUME_PDQ_AMD_EINVAL;
+  if (force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold)))
+    return YUME_PDQ_AMD_MATCH_MFMA;
+  return YUME_PDQ_AMD_MATCH_VALU;
+}
+
 int yume_pdq_amd_match_first(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,
                              uint32_t threshold, unsigned long long* d_first, void* stream) {
   if (!d_first || (n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;

@@ -68,6 +68,8 @@ def lib():
         L.yume_pdq_amd_match.argtypes = [vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
         L.yume_pdq_amd_match_first.restype = i
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
+        L.yume_pdq_amd_match_path.restype = i
+        L.yume_pdq_amd_match_path.argtypes = [u32, u32, u32]
         for name, args in (("compress", [vp, i, u64, u32, vp, vp]),
                            ("dct2d", [vp, u32, vp, vp, vp]),
                            ("sum_of_gradients", [vp, u32, vp, vp]),
@@ -250,6 +252,14 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
     arr = hits[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
     arr = np.sort(arr, order=("index", "query"))
     return total, arr
+
+
+def match_uses_matrix_cores(n_queries, threshold, path=None):
+    """True when match() runs the matrix-core kernel for these arguments."""
+    flags = {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path]
+    rc = lib().yume_pdq_amd_match_path(int(n_queries), int(threshold), flags)
+    _check(rc, "match_path")
+    return rc == MATCH_MFMA
 
 
 def match_first(db, queries, threshold, stream=None):
