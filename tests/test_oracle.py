@@ -337,3 +337,75 @@ def test_matcher_brute_force_closed_form(oracle):
         assert oracle.hamming(db[r], q[qi]) == d
     # inclusive threshold (mod.rs:251,354)
     assert oracle.match_all(db[:1], db[:1], 0)[0] == 1
+
+
+# --- precise (reference) kernel and the FP64 oracle -------------------------------------
+
+def _tent_closed_form_f64(img, centres, w):
+    """sum_a sum_b w[a] w[b] img[c_i + a - r][c_j + b - r] over the given centres."""
+    r = len(w) // 2
+    img = img.astype(np.float64)
+    out = np.zeros((len(centres), len(centres)))
+    W = np.outer(w, w)
+    for i, ci in enumerate(centres):
+        for j, cj in enumerate(centres):
+            out[i, j] = (W * img[ci - r:ci + r + 1, cj - r:cj + r + 1]).sum()
+    return out
+
+
+def test_pdqf_f64_closed_form(oracle):
+    """The FP64 oracle (tent at centres 4(i+1), weights k/81, f64 DCT) equals
+    the closed form D64 C D64^T evaluated independently in numpy."""
+    _, d64, _ = oracle.tables()
+    D = d64[:16 * 127].reshape(16, 127)
+    img = _images(1)[0]
+    C = _tent_closed_form_f64(img, [4 * (i + 1) for i in range(127)],
+                              np.array([1, 2, 3, 2, 1], np.float64) / 9.0)
+    exact = D @ C @ D.T
+    got = oracle.pdqf_f64(img)
+    assert np.abs(got - exact).max() <= 1e-12 * np.abs(exact).max()
+
+
+def test_reference_kernel_closed_form(oracle):
+    """ReferenceKernel (mod.rs:790-820, ref.rs:7-194): two Jarosz passes of a
+    width-3 box in each direction are the separable (1,2,3,2,1)/9 filter away
+    from the border; sampled at floor((2i+1) 512 / 254) every sample is
+    interior, so the f64 kernel's PDQF is D64 C D64^T of that closed form to
+    f64 rounding, and the f32 kernel's to f32 accumulation error.  The bits
+    of both follow the lower-median threshold (mod.rs:764-788)."""
+    _, d64, _ = oracle.tables()
+    D = d64[:16 * 127].reshape(16, 127)
+    for img in _images(2):
+        centres = [((2 * i + 1) * 512) // 254 for i in range(127)]
+        C = _tent_closed_form_f64(img, centres, np.array([1, 2, 3, 2, 1], np.float64) / 9.0)
+        exact = D @ C @ D.T
+        h64, t64, q64, p64 = oracle.hash_ref64(img)
+        assert np.abs(p64 - exact).max() <= 1e-11 * np.abs(exact).max()
+        h32, t32, q32, p32 = oracle.hash_ref32(img)
+        assert np.abs(p32 - exact).max() <= 1e-4 * np.abs(exact).max()
+        for h, p, t in ((h64, p64, t64), (h32, p32, t32)):
+            flat = p.reshape(-1)
+            assert t == np.sort(flat)[127]
+            exp = np.zeros(32, np.uint8)
+            for i in range(256):
+                if flat[i] > t:
+                    exp[31 - i // 8] |= 1 << (i % 8)
+            assert np.array_equal(h, exp)
+        assert bits(h32, h64) <= 8
+
+
+def test_scalar_dct_matches_fp64_closed_form(oracle):
+    """DefaultKernel::dct2d (mod.rs:543-574) restated in the oracle equals
+    D C D^T from the f64 table within f32 accumulation error, and agrees with
+    the AVX2 restatement to the same order."""
+    _, d64, _ = oracle.tables()
+    D = d64[:16 * 127].reshape(16, 127)
+    rng = np.random.default_rng(29)
+    buf = np.zeros((128, 128), np.float32)
+    buf[:127, :127] = rng.uniform(0, 255, (127, 127)).astype(np.float32)
+    exact = D @ buf[:127, :127].astype(np.float64) @ D.T
+    sc = oracle.dct2d_scalar(buf)
+    av, _ = oracle.dct2d_avx2(buf)
+    scale = np.abs(exact).max()
+    assert np.abs(sc - exact).max() <= 2e-5 * scale
+    assert np.abs(sc - av).max() <= 4e-5 * scale
