@@ -247,6 +247,9 @@ def main():
     dom_is_match = t_match_head >= t_hash_head
     roofline = roofline_match if dom_is_match else roofline_hash
 
+    log("[bench] drop-in FFI: yume_pdq_hash_smart_kernel on a host f32 image")
+    ffi_us = drop_in_ffi_us(yume_pdq, planes_u8[0].to(torch.float32).cpu().numpy())
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, db, thr, n_rows)
@@ -287,8 +290,35 @@ def main():
         "plain_unbatched_f32": plain,
         "speedup_vs_plain_unbatched": round(head["images_per_s"] / plain["images_per_s"], 2),
         "cpu_baseline": cpu,
+        "drop_in_ffi": {
+            "us_per_hash": round(ffi_us, 2),
+            "note": ("PCIe-inclusive per-image rate of the reference ABI: 1 MiB f32 host image "
+                     "copied in, fused kernel, hash/threshold/PDQF/buffers copied out, per call "
+                     "through ctypes with preallocated buffers (as integration/hash.py)"),
+        },
     }
     print(json.dumps(line), flush=True)
+
+
+def drop_in_ffi_us(yume_pdq, image, n=300):
+    """Mean microseconds per yume_pdq_hash_smart_kernel call on a host image."""
+    import ctypes as C
+
+    import numpy as np
+    L = yume_pdq.lib()
+    img = np.ascontiguousarray(image, np.float32)
+    thr = np.zeros(1, np.float32)
+    out = np.zeros(32, np.uint8)
+    buf1 = np.zeros((128, 128), np.float32)
+    tmp = np.zeros(128, np.float32)
+    pdqf = np.zeros((16, 16), np.float32)
+    args = [x.ctypes.data_as(C.c_void_p) for x in (img, thr, out, buf1, tmp, pdqf)]
+    for _ in range(20):
+        L.yume_pdq_hash_smart_kernel(*args)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        L.yume_pdq_hash_smart_kernel(*args)
+    return (time.perf_counter() - t0) * 1e6 / n
 
 
 def load_counter_summary():

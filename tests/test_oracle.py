@@ -409,3 +409,21 @@ def test_scalar_dct_matches_fp64_closed_form(oracle):
     scale = np.abs(exact).max()
     assert np.abs(sc - exact).max() <= 2e-5 * scale
     assert np.abs(sc - av).max() <= 4e-5 * scale
+
+
+def test_fp32_path_error_vs_fp64(oracle):
+    """The error statement for the FP32 path (which the GPU kernel reproduces
+    bit for bit): on synthetic images every PDQF coefficient is within
+    1e-6 x max|coefficient| of the FP64 oracle, and the hash equals the hash
+    of the FP64 PDQF thresholded at its lower median (0 bits disagree)."""
+    for img in _images(16):
+        h, _, _, p, _, _ = oracle.hash_get_threshold(img)
+        p64 = oracle.pdqf_f64(img)
+        assert np.abs(p.astype(np.float64) - p64).max() <= 1e-6 * np.abs(p64).max()
+        flat = p64.reshape(-1)
+        t = np.sort(flat)[127]
+        h64 = np.zeros(32, np.uint8)
+        for i in range(256):
+            if flat[i] > t:
+                h64[31 - i // 8] |= 1 << (i % 8)
+        assert bits(h, h64) == 0
