@@ -113,6 +113,25 @@ int yume_pdq_amd_match(const uint8_t *d_db, uint64_t n_rows, const uint8_t *d_qu
                        uint64_t capacity, unsigned long long *d_count, uint64_t index_base,
                        uint32_t flags, void *stream);
 
+/*
+ * Prefix plane: a second, matcher-ordered copy of the first 16 bytes of every
+ * database row (16-row tiles stored word-major, zero-padded to whole tiles),
+ * built once per database (and again after rows change).  With it the
+ * matrix-core matcher reads 256 contiguous bytes per 16-row tile instead of
+ * 16 scattered half-rows: half the bytes and no partially used cache lines.
+ * No reference counterpart (a device-side index of the database; the
+ * reference's VulkanMatcher likewise keeps its own device copy,
+ * src/matching/vulkan.rs:40-64).
+ */
+uint64_t yume_pdq_amd_prefix_plane_bytes(uint64_t n_rows);
+int yume_pdq_amd_build_prefix_plane(const uint8_t *d_db, uint64_t n_rows, void *d_plane, void *stream);
+/* yume_pdq_amd_match using the prefix plane of d_db (same results; the plane
+ * must have been built from the current rows). */
+int yume_pdq_amd_match_with_plane(const uint8_t *d_db, const void *d_plane, uint64_t n_rows,
+                                  const uint8_t *d_queries, uint32_t n_queries, uint32_t threshold,
+                                  YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
+                                  uint64_t index_base, uint32_t flags, void *stream);
+
 /* Which kernel yume_pdq_amd_match would run for these arguments:
  * YUME_PDQ_AMD_MATCH_VALU or YUME_PDQ_AMD_MATCH_MFMA (or EINVAL for
  * contradictory flags).  No reference counterpart (path introspection for

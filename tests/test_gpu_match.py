@@ -61,8 +61,15 @@ def test_planted_controls(gpu, oracle, dev, nq):
         assert first == (int(pos), i)
 
 
-THRESHOLD_CASES = [(t, p) for t in (0, 1, 31, 63, 64, 128, 200, 255, 256) for p in ("valu", "mfma")
-                   if not (p == "mfma" and t >= 64)]
+THRESHOLD_CASES = [(t, p) for t in (0, 1, 31, 63, 64, 128, 200, 255, 256) for p in ("valu", "mfma", "mfma_plane")
+                   if not (p.startswith("mfma") and t >= 64)]
+
+
+def run_path(gpu, d_db, d_q, threshold, path, **kw):
+    """path: valu / mfma (raw rows) / mfma_plane (matrix cores over the prefix plane)."""
+    if path == "mfma_plane":
+        return gpu.match(d_db, d_q, threshold, path="mfma", plane=gpu.build_prefix_plane(d_db), **kw)
+    return gpu.match(d_db, d_q, threshold, path=path, **kw)
 
 
 @pytest.mark.parametrize("threshold,path", THRESHOLD_CASES)
@@ -73,25 +80,44 @@ def test_thresholds(gpu, oracle, dev, threshold, path):
     synthetic.plant(db, q, 200, seed=5, max_dist=70)
     say(f"[match] threshold={threshold} path={path}")
     exp_n, exp = oracle.match_all(db, q, threshold, cap=3000 * 40)
-    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), threshold,
-                            capacity=3000 * 40, path=path)
+    total, hits = run_path(gpu, torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), threshold, path,
+                           capacity=3000 * 40)
     assert total == exp_n
     assert np.array_equal(sort_triples(as_triples(hits)), exp)
 
 
+@pytest.mark.parametrize("plane", [False, True])
 @pytest.mark.parametrize("nq", [1, 15, 16, 17, 33, 100, 257, 1000])
-def test_mfma_query_counts(gpu, oracle, dev, nq):
+def test_mfma_query_counts(gpu, oracle, dev, nq, plane):
     """Ragged query counts on the matrix-core path (partial 16-query tiles,
-    partial query groups) against the oracle, ragged row counts too."""
+    partial query groups) against the oracle, ragged row counts too (5003
+    rows: a partial last tile and a partial last wave), raw rows and plane."""
     import torch
     db = synthetic.random_db(5003, seed=40 + nq)
     q = synthetic.random_db(nq, seed=41)
     synthetic.plant(db, q, 300, seed=42, max_dist=45)
-    say(f"[match] mfma nq={nq}")
+    say(f"[match] mfma nq={nq} plane={plane}")
     exp_n, exp = oracle.match_all(db, q, 31)
-    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 31, path="mfma")
+    total, hits = run_path(gpu, torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 31,
+                           "mfma_plane" if plane else "mfma")
     assert total == exp_n
     assert np.array_equal(sort_triples(as_triples(hits)), exp)
+
+
+def test_prefix_plane_layout(gpu, dev):
+    """plane[tile * 64 + kq * 16 + rn] = word kq of row 16 tile + rn, zero past
+    the last row (checked on the host against numpy)."""
+    import torch
+    for n in (1, 15, 16, 17, 1000):
+        db = synthetic.random_db(n, seed=n)
+        say(f"[match] prefix plane layout n={n}")
+        plane = gpu.build_prefix_plane(torch.from_numpy(db).to(dev)).cpu().numpy().view(np.uint32)
+        tiles = (n + 15) // 16
+        assert plane.size == tiles * 64
+        words = np.zeros((tiles * 16, 8), np.uint32)
+        words[:n] = db.view(np.uint32).reshape(n, 8)
+        exp = words[:, :4].reshape(tiles, 16, 4).transpose(0, 2, 1).reshape(-1)
+        assert np.array_equal(plane, exp)
 
 
 def test_empty_and_tiny(gpu, dev):
@@ -166,6 +192,9 @@ def test_scale_10m_x_1024_paths_agree(gpu, dev):
     say("[match] 10M x 1024 valu")
     n2, h2 = gpu.match(d_db, d_q, 31, capacity=1 << 20, path="valu")
     assert n1 == n2 and np.array_equal(as_triples(h1), as_triples(h2))
+    say("[match] 10M x 1024 mfma over the prefix plane")
+    n3, h3 = gpu.match(d_db, d_q, 31, capacity=1 << 20, path="mfma", plane=gpu.build_prefix_plane(d_db))
+    assert n3 == n1 and np.array_equal(as_triples(h3), as_triples(h1))
     assert n1 >= 0.9 * 50_000 * 32 / 41  # plants at d <= 31 are hits
     sub = synthetic.brute_force_hits(db[:500_000], q, 31)
     t1 = as_triples(h1)

@@ -5,6 +5,8 @@
 //                   scale for the FETCH_SIZE / WRITE_SIZE counters)
 //   match_x8        exhaustive match of one image's 8 dihedral hashes
 //   match_x1024     exhaustive match of 128 images' 1024 dihedral hashes
+//   match_plane_x1024  the same over the database's prefix plane
+//   build_plane     building the prefix plane (not in the default set)
 //   hash_u8         fused hash of --batch u8 planes
 // The database holds 50,000 planted near-duplicates (U{0..40} distinct bit
 // flips) drawn from a pool of 8192 hashes whose first 1024 are the queries.
@@ -76,7 +78,7 @@ int main(int argc, char** argv) {
   uint64_t rows = 10'000'000;
   int steps = 20, warmup = 3;
   uint32_t batch = 128;
-  std::string cases = "copy_db,match_x8,match_x1024,hash_u8";
+  std::string cases = "copy_db,match_x8,match_x1024,match_plane_x1024,hash_u8";
   for (int i = 1; i + 1 < argc; i += 2) {
     const std::string a = argv[i];
     if (a == "--rows") rows = std::strtoull(argv[i + 1], nullptr, 10);
@@ -171,6 +173,17 @@ int main(int argc, char** argv) {
                    api_ok(yume_pdq_amd_match(d_db, rows, d_hashes, 1024, 31, d_hits, cap, d_count, 0, 0, s),
                           "match");
                  }});
+  void* d_plane = nullptr;
+  hip_ok(hipMalloc(&d_plane, yume_pdq_amd_prefix_plane_bytes(rows)), "hipMalloc plane");
+  api_ok(yume_pdq_amd_build_prefix_plane(d_db, rows, d_plane, s), "build_prefix_plane");
+  all.push_back({"build_plane", [&] {
+                   api_ok(yume_pdq_amd_build_prefix_plane(d_db, rows, d_plane, s), "build_prefix_plane");
+                 }});
+  all.push_back({"match_plane_x1024", [&] {
+                   api_ok(yume_pdq_amd_match_with_plane(d_db, d_plane, rows, d_hashes, 1024, 31, d_hits, cap, d_count,
+                                                        0, 0, s),
+                          "match_with_plane");
+                 }});
   all.push_back({"hash_u8", [&] {
                    api_ok(yume_pdq_amd_hash_batch(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, batch,
                                                   YUME_PDQ_AMD_DIHEDRAL_EXACT, d_hashes, d_q, d_t, nullptr, s),
@@ -213,5 +226,6 @@ int main(int argc, char** argv) {
   (void)hipFree(d_t);
   (void)hipFree(d_hits);
   (void)hipFree(d_count);
+  (void)hipFree(d_plane);
   return 0;
 }

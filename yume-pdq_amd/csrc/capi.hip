@@ -256,10 +256,11 @@ int yume_pdq_amd_dihedrals_batch(const float* d_pdqf, const float* d_thr_in, uin
   return from_hip(ypdq::launch_dihedrals(d_pdqf, d_thr_in, n, dihedral_mode, o, static_cast<hipStream_t>(stream)));
 }
 
-int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,
-                       uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity, unsigned long long* d_count,
-                       uint64_t index_base, uint32_t flags, void* stream) {
+static int match_impl(const uint8_t* d_db, const void* d_plane, uint64_t n_rows, const uint8_t* d_queries,
+                      uint32_t n_queries, uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity,
+                      unsigned long long* d_count, uint64_t index_base, uint32_t flags, void* stream) {
   if (!d_count || (capacity && !d_hits)) return YUME_PDQ_AMD_EINVAL;
+  if ((uintptr_t)d_plane % 16) return YUME_PDQ_AMD_EINVAL;
   if ((n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;
   if ((uintptr_t)d_db % 16 || (uintptr_t)d_queries % 4) return YUME_PDQ_AMD_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -276,10 +277,33 @@ int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_qu
   if (force_mfma && threshold >= 64) return YUME_PDQ_AMD_EINVAL;
   const bool mfma = force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold));
   if (mfma)
-    return from_hip(ypdq::launch_match_mfma(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count,
-                                            index_base, st));
+    return from_hip(ypdq::launch_match_mfma(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
+                                            d_count, index_base, st));
   return from_hip(
       ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count, index_base, st));
+}
+
+int yume_pdq_amd_match(const uint8_t* d_db, uint64_t n_rows, const uint8_t* d_queries, uint32_t n_queries,
+                       uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity, unsigned long long* d_count,
+                       uint64_t index_base, uint32_t flags, void* stream) {
+  return match_impl(d_db, nullptr, n_rows, d_queries, n_queries, threshold, d_hits, capacity, d_count, index_base,
+                    flags, stream);
+}
+
+int yume_pdq_amd_match_with_plane(const uint8_t* d_db, const void* d_plane, uint64_t n_rows, const uint8_t* d_queries,
+                                  uint32_t n_queries, uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity,
+                                  unsigned long long* d_count, uint64_t index_base, uint32_t flags, void* stream) {
+  if (!d_plane && n_rows) return YUME_PDQ_AMD_EINVAL;
+  return match_impl(d_db, d_plane, n_rows, d_queries, n_queries, threshold, d_hits, capacity, d_count, index_base,
+                    flags, stream);
+}
+
+uint64_t yume_pdq_amd_prefix_plane_bytes(uint64_t n_rows) { return ypdq::prefix_plane_bytes(n_rows); }
+
+int yume_pdq_amd_build_prefix_plane(const uint8_t* d_db, uint64_t n_rows, void* d_plane, void* stream) {
+  if (n_rows == 0) return YUME_PDQ_AMD_OK;
+  if (!d_db || !d_plane || (uintptr_t)d_db % 16 || (uintptr_t)d_plane % 16) return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_build_prefix_plane(d_db, n_rows, d_plane, static_cast<hipStream_t>(stream)));
 }
 
 int yume_pdq_amd_match_path(uint32_t n_queries, uint32_t threshold, uint32_t flags) {

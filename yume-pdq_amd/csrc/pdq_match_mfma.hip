@@ -67,14 +67,16 @@ __device__ __forceinline__ uint32_t full_distance(const uint32_t* __restrict__ a
 }
 
 // Exact check of query tile t's candidates in this lane's four outputs.
-__device__ __noinline__ void resolve_candidates(const float* cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
+__device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
                                                 float bound, const uint32_t* __restrict__ queries,
                                                 const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
                                                 uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
                                                 unsigned long long* __restrict__ counter, uint64_t index_base) {
   const uint64_t row = tbase + rn;
+#pragma unroll
   for (int r = 0; r < 4; r++) {
-    if (cv[r] >= bound) {
+    const float v = r == 0 ? cv.x : r == 1 ? cv.y : r == 2 ? cv.z : cv.w;
+    if (v >= bound) {
       const uint32_t qi = qbase + 16 * t + 4 * kq + r;
       if (qi < n_queries && row < n_rows) {
         const uint32_t d = full_distance(queries + (size_t)qi * 8, db + row * 8);
@@ -93,22 +95,14 @@ __device__ __noinline__ void resolve_candidates(const float* cv, int t, uint64_t
   }
 }
 
-// One 16-row tile against every query tile of the wave.  The dot products are
-// reduced to a running maximum as they come out of the matrix core; in the
-// rare case the maximum clears the bound, the tile's products are recomputed
-// and every surviving pair is re-checked with the exact 256-bit distance.
+// Maximum over one 16-row tile of the dot products with every query tile of
+// the wave.  The products are integers in [-128, 128] and the bound is
+// positive, so comparing float bit patterns as signed integers orders them
+// exactly (negative floats are negative integers); integer max3 needs no NaN
+// canonicalisation.  (Element-wise __float_as_int: a bit cast of a vector
+// element expression is miscompiled to element 0 by this toolchain.)
 template <int QT>
-__device__ __forceinline__ void tile_check(const Expanded (&qa)[QT], const Expanded& rb, uint64_t tbase, uint32_t qbase,
-                                           int rn, int kq, float bound, const uint32_t* __restrict__ queries,
-                                           const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
-                                           uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
-                                           unsigned long long* __restrict__ counter, uint64_t index_base) {
-  // The products are integers in [-128, 128] and the bound is positive, so
-  // comparing float bit patterns as signed integers orders them exactly
-  // (negative floats are negative integers); integer max3 needs no NaN
-  // canonicalisation.  (Element-wise __float_as_int: a bit cast of a vector
-  // element expression is miscompiled to element 0 by this toolchain.)
-  int mx = INT32_MIN;
+__device__ __forceinline__ int tile_max(const Expanded (&qa)[QT], const Expanded& rb, int mx) {
 #pragma unroll
   for (int t = 0; t < QT; t++) {
     const v4f c = mfma_prefix(qa[t], rb);
@@ -117,50 +111,97 @@ __device__ __forceinline__ void tile_check(const Expanded (&qa)[QT], const Expan
     mx = max(max(mx, c0), c1);
     mx = max(max(mx, c2), c3);
   }
-  if (__builtin_expect(wave_any(mx >= __float_as_int(bound)), 0)) {
-    for (int t = 0; t < QT; t++) {
-      const v4f c = mfma_prefix(qa[t], rb);
-      float cv[4] = {c[0], c[1], c[2], c[3]};
-      resolve_candidates(cv, t, tbase, qbase, rn, kq, bound, queries, db, n_queries, n_rows, threshold, hits,
-                         capacity, counter, index_base);
-    }
+  return mx;
+}
+
+// The rare path: recompute one tile's products and re-check every pair that
+// clears the bound with the exact 256-bit distance.
+template <int QT>
+__device__ __forceinline__ void resolve_tile(const Expanded (&qa)[QT], const Expanded& rb, uint64_t tbase,
+                                             uint32_t qbase, int rn, int kq, float bound,
+                                             const uint32_t* __restrict__ queries, const uint32_t* __restrict__ db,
+                                             uint32_t n_queries, uint64_t n_rows, uint32_t threshold,
+                                             MatchHit* __restrict__ hits, uint64_t capacity,
+                                             unsigned long long* __restrict__ counter, uint64_t index_base) {
+  for (int t = 0; t < QT; t++) {
+    const v4f c = mfma_prefix(qa[t], rb);
+    resolve_candidates(c, t, tbase, qbase, rn, kq, bound, queries, db, n_queries, n_rows, threshold, hits, capacity,
+                       counter, index_base);
   }
 }
 
-template <int QT, bool kFull>
+// Where a wave's row words come from.  Raw rows: lane (rn, kq) reads word kq
+// of row r at db[r * 8 + kq] -- 16 of every 32 bytes, scattered over 4 cache
+// lines per tile.  Prefix plane: the same words stored tile-major,
+// plane[tile * 64 + kq * 16 + rn], so a tile is 256 contiguous bytes and a
+// launch reads half the bytes.
+template <bool kPlane>
+struct RowSource {
+  const uint32_t* __restrict__ p;
+  __device__ __forceinline__ RowSource(const uint32_t* db, const uint32_t* plane, uint64_t row0, int lane)
+      : p(kPlane ? plane + (row0 / 16) * 64 + lane : db + (row0 + (lane & 15)) * 8 + (lane >> 4)) {}
+  __device__ __forceinline__ uint32_t operator()(uint32_t tile) const {
+    return p[(size_t)tile * (kPlane ? 64 : 128)];
+  }
+};
+
+// Stream the wave's tiles: loads run kAhead tiles ahead, the next tile's
+// operand is expanded while the current tile's products run, and the
+// bound is tested once per kGroup tiles (a wave-uniform branch that is taken
+// about once per million tiles on random data).
+template <int QT, bool kFull, bool kPlane>
 __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uint32_t* __restrict__ lut,
-                                             const uint32_t* __restrict__ db, uint64_t row0, uint64_t n_rows,
-                                             uint32_t qbase, int rn, int kq, float bound,
-                                             const uint32_t* __restrict__ queries, uint32_t n_queries,
+                                             const RowSource<kPlane>& src, const uint32_t* __restrict__ db,
+                                             uint64_t row0, uint64_t n_rows, uint32_t qbase, int rn, int kq,
+                                             float bound, const uint32_t* __restrict__ queries, uint32_t n_queries,
                                              uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
                                              unsigned long long* __restrict__ counter, uint64_t index_base) {
-  constexpr int kAhead = 4;
-  // this lane's word of row (row0 + off): db_lane[off * 8]
-  const uint32_t* __restrict__ db_lane = db + (row0 + rn) * 8 + kq;
-  const uint32_t left = kFull ? 0u : (uint32_t)(n_rows - row0);  // rows of this wave that exist
+  constexpr int kAhead = 8;
+  constexpr int kGroup = kFull ? 4 : 1;
+  static_assert(kMfmaTilesPerWave % kAhead == 0 && kAhead % kGroup == 0, "tile loop shape");
+  // rows of this wave that exist (the last wave of the database only)
+  const uint32_t left = kFull ? 0u : (uint32_t)(n_rows - row0);
+  const uint32_t tiles = kFull ? (uint32_t)kMfmaTilesPerWave : (left + 15) / 16;
   auto fetch = [&](uint32_t tile) -> uint32_t {
-    const uint32_t off = tile * 16;
-    if (kFull) return db_lane[(size_t)off * 8];
-    return (off + rn < left) ? db_lane[(size_t)off * 8] : 0u;
+    if (kFull || kPlane) return src(tile);  // the plane is padded to whole tiles
+    return (tile * 16 + rn < left) ? src(tile) : 0u;
   };
   uint32_t ring[kAhead];
 #pragma unroll
-  for (int a = 0; a < kAhead; a++) ring[a] = fetch(a);
+  for (int a = 0; a < kAhead; a++) ring[a] = fetch(a < (int)tiles ? a : tiles - 1);
   Expanded cur;
   expand_word(ring[0], lut, cur);
-  const uint32_t tiles = kFull ? (uint32_t)kMfmaTilesPerWave : (left + 15) / 16;
+  uint32_t held[kGroup];  // raw words of the tiles since the last test
+  int mx = INT32_MIN;
+  const int ibound = __float_as_int(bound);
 #pragma unroll 1
   for (uint32_t tile = 0; tile < tiles; tile += kAhead) {
 #pragma unroll
     for (int a = 0; a < kAhead; a++) {
       const uint32_t t = tile + a;
-      ring[a] = (t + kAhead < tiles) ? fetch(t + kAhead) : 0u;
       if (!kFull && t >= tiles) break;
+      held[a % kGroup] = ring[a];
+      // Always issue the load (the last kAhead tiles re-read tile `tiles - 1`
+      // and drop it): a load under a branch makes the wait counters unknown
+      // to the compiler, which then waits for every load in flight.
+      ring[a] = fetch(t + kAhead < tiles ? t + kAhead : tiles - 1);
       Expanded rb = cur;
       if (!kFull && t * 16 + rn >= left) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
       expand_word(ring[(a + 1) % kAhead], lut, cur);  // operand of tile t + 1
-      tile_check<QT>(qa, rb, row0 + (uint64_t)t * 16, qbase, rn, kq, bound, queries, db, n_queries, n_rows,
-                     threshold, hits, capacity, counter, index_base);
+      mx = tile_max<QT>(qa, rb, mx);
+      if (a % kGroup == kGroup - 1) {
+        if (__builtin_expect(wave_any(mx >= ibound), 0)) {
+          for (int g = 0; g < kGroup; g++) {
+            const uint32_t tg = t + 1 - kGroup + g;
+            Expanded eg;
+            expand_word(held[g], lut, eg);
+            if (!kFull && tg * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
+            resolve_tile<QT>(qa, eg, row0 + (uint64_t)tg * 16, qbase, rn, kq, bound, queries, db, n_queries, n_rows,
+                             threshold, hits, capacity, counter, index_base);
+          }
+        }
+        mx = INT32_MIN;
+      }
     }
   }
 }
@@ -171,11 +212,11 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
 // own L2; block b is mapped so that all query groups of a row chunk run on
 // the same XCD (row chunk = xcd + 8 * (slot / groups)) and read its rows from
 // that XCD's L2 after the first one brought them in from HBM.
-template <int QT>
+template <int QT, bool kPlane>
 __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_kernel(
-    const uint32_t* __restrict__ db, uint64_t n_rows, const uint32_t* __restrict__ queries, uint32_t n_queries,
-    uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter,
-    uint64_t index_base, uint32_t groups) {
+    const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
+    const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
+    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
   __shared__ uint32_t lut[256];
   lut[threadIdx.x] = spread_byte(threadIdx.x);  // blockDim.x == 256
   __syncthreads();
@@ -191,72 +232,106 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 
   // query operands: tile t, query qbase + 16t + rn, word kq
   const uint32_t qbase = group * (uint32_t)(QT * 16);
-  Expanded qa[QT];
+  // All QT words are loaded before any is used (loads clamped in bounds rather
+  // than branched around, so they overlap instead of waiting one by one).
+  uint32_t qw[QT];
 #pragma unroll
   for (int t = 0; t < QT; t++) {
     const uint32_t qi = qbase + 16 * t + rn;
-    if (qi < n_queries) {
-      expand_word(queries[(size_t)qi * 8 + kq], lut, qa[t]);
-    } else {
+    qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+  }
+  Expanded qa[QT];
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    expand_word(qw[t], lut, qa[t]);
+    if (qbase + 16 * t + rn >= n_queries)
       qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;  // zero rows never reach the bound
-    }
   }
 
   const uint64_t wave_rows = (uint64_t)kMfmaTilesPerWave * 16;
   const uint64_t row0 = (chunk * kMfmaWaves + wave) * wave_rows;
   if (row0 >= n_rows) return;
-
-  // Rows stream in four tiles ahead (one 32-bit word per lane per tile); the
-  // next tile's operand is expanded while the current tile's products run.
+  const RowSource<kPlane> src(db, plane, row0, lane);
   // Only the wave that reaches the end of the database needs bounds checks.
   if (row0 + wave_rows <= n_rows)
-    stream_tiles<QT, true>(qa, lut, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries, threshold, hits,
-                           capacity, counter, index_base);
+    stream_tiles<QT, true, kPlane>(qa, lut, src, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries,
+                                   threshold, hits, capacity, counter, index_base);
   else
-    stream_tiles<QT, false>(qa, lut, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries, threshold, hits,
-                            capacity, counter, index_base);
+    stream_tiles<QT, false, kPlane>(qa, lut, src, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries,
+                                    threshold, hits, capacity, counter, index_base);
 }
 
 template <int QT>
-static hipError_t launch_qt(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
-                            uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                            uint64_t index_base, hipStream_t stream) {
+static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                            uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                            unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
   const uint64_t rows_per_block = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
   const uint64_t chunks = (n_rows + rows_per_block - 1) / rows_per_block;
   const uint64_t chunks8 = (chunks + 7) / 8 * 8;  // whole rounds over the 8 XCDs
   const uint32_t groups = (n_queries + QT * 16 - 1) / (QT * 16);
   const uint64_t blocks = chunks8 * groups;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((pdq_match_mfma_kernel<QT>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
-                     static_cast<const uint32_t*>(db), n_rows, static_cast<const uint32_t*>(queries), n_queries,
-                     threshold, hits, capacity, counter, index_base, groups);
+  if (plane)
+    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
+                       static_cast<const uint32_t*>(db), static_cast<const uint32_t*>(plane), n_rows,
+                       static_cast<const uint32_t*>(queries), n_queries, threshold, hits, capacity, counter,
+                       index_base, groups);
+  else
+    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
+                       static_cast<const uint32_t*>(db), static_cast<const uint32_t*>(plane), n_rows,
+                       static_cast<const uint32_t*>(queries), n_queries, threshold, hits, capacity, counter,
+                       index_base, groups);
+  return hipGetLastError();
+}
+
+// Prefix plane: for tile T (rows 16T .. 16T+15) and lane L = kq * 16 + rn,
+// plane[T * 64 + L] = word kq of row 16T + rn (0 past the last row).
+__global__ void build_prefix_plane_kernel(const uint32_t* __restrict__ db, uint64_t n_rows,
+                                          uint32_t* __restrict__ plane, uint64_t n_words) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_words) return;
+  const uint64_t tile = i >> 6;
+  const uint32_t lane = (uint32_t)(i & 63);
+  const uint64_t row = tile * 16 + (lane & 15);
+  plane[i] = row < n_rows ? db[row * 8 + (lane >> 4)] : 0u;
+}
+
+uint64_t prefix_plane_bytes(uint64_t n_rows) { return (n_rows + 15) / 16 * 256; }
+
+hipError_t launch_build_prefix_plane(const void* db, uint64_t n_rows, void* plane, hipStream_t stream) {
+  const uint64_t n_words = (n_rows + 15) / 16 * 64;
+  if (n_words == 0) return hipSuccess;
+  const uint64_t blocks = (n_words + 255) / 256;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(build_prefix_plane_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                     static_cast<const uint32_t*>(db), n_rows, static_cast<uint32_t*>(plane), n_words);
   return hipGetLastError();
 }
 
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold) { return threshold < 64 && n_queries >= 32; }
 
-hipError_t launch_match_mfma_qt(int max_qt, const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
-                                uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                                uint64_t index_base, hipStream_t stream) {
+hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                                uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
   if (n_rows == 0 || n_queries == 0) return hipSuccess;
   if (threshold >= 64) return hipErrorInvalidValue;
   const uint32_t tiles = (n_queries + 15) / 16;
   if (tiles <= 2 || max_qt == 2)
-    return launch_qt<2>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<2>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
   if (tiles <= 4 || max_qt == 4)
-    return launch_qt<4>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<4>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
   if (tiles <= 8 || max_qt == 8)
-    return launch_qt<8>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
-  return launch_qt<16>(db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<8>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+  return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
 }
 
-hipError_t launch_match_mfma(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,
-                             uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                             uint64_t index_base, hipStream_t stream) {
+hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                             uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                             unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
   // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
   // 1024 / 8192 queries); larger tiles cost occupancy
-  return launch_match_mfma_qt(8, db, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
-                              stream);
+  return launch_match_mfma_qt(8, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
+                              index_base, stream);
 }
 
 }  // namespace ypdq

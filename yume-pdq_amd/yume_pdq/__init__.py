@@ -68,6 +68,12 @@ def lib():
         L.yume_pdq_amd_match.argtypes = [vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
         L.yume_pdq_amd_match_first.restype = i
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
+        L.yume_pdq_amd_match_with_plane.restype = i
+        L.yume_pdq_amd_match_with_plane.argtypes = [vp, vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
+        L.yume_pdq_amd_prefix_plane_bytes.restype = u64
+        L.yume_pdq_amd_prefix_plane_bytes.argtypes = [u64]
+        L.yume_pdq_amd_build_prefix_plane.restype = i
+        L.yume_pdq_amd_build_prefix_plane.argtypes = [vp, u64, vp, vp]
         L.yume_pdq_amd_match_path.restype = i
         L.yume_pdq_amd_match_path.argtypes = [u32, u32, u32]
         for name, args in (("compress", [vp, i, u64, u32, vp, vp]),
@@ -220,11 +226,25 @@ def dihedrals_batch(pdqf, thr_in, dihedral_mode=DIHEDRAL_EXACT, stream=None):
 HIT_DTYPE = np.dtype([("query", "<u4"), ("distance", "<u4"), ("index", "<u8")])
 
 
+def build_prefix_plane(db, stream=None):
+    """The database's prefix plane (a u8 device tensor) for match(..., plane=)."""
+    torch = _torch()
+    db = db.reshape(-1, 32)
+    if not db.is_contiguous():
+        raise ValueError("db must be contiguous")
+    n = db.shape[0]
+    plane = torch.empty(int(lib().yume_pdq_amd_prefix_plane_bytes(n)), dtype=torch.uint8, device=db.device)
+    _check(lib().yume_pdq_amd_build_prefix_plane(C.c_void_p(db.data_ptr()), n, C.c_void_p(plane.data_ptr()),
+                                                 _stream_ptr(stream)), "build_prefix_plane")
+    return plane
+
+
 def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, hits=None,
-          count=None, append=False, sync=True, path=None):
+          count=None, append=False, sync=True, path=None, plane=None):
     """All (query, index, distance) with popcount(query ^ row) <= threshold.
 
-    db: CUDA u8 tensor (n_rows, 32); queries: CUDA u8 tensor (n_queries, 32).
+    db: CUDA u8 tensor (n_rows, 32); queries: CUDA u8 tensor (n_queries, 32);
+    plane: the prefix plane of db (build_prefix_plane) or None.
     Returns (total_count, hits ndarray of HIT_DTYPE sorted by (index, query))
     when sync, else the device (hits, count) tensors."""
     torch = _torch()
@@ -240,11 +260,17 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
     if count is None:
         count = torch.zeros(1, dtype=torch.int64, device=dev)
     capacity = hits.shape[0]
-    _check(lib().yume_pdq_amd_match(
-        C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
-        int(threshold), C.c_void_p(hits.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
-        index_base, (MATCH_APPEND if append else 0) | {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path],
-        _stream_ptr(stream)), "match")
+    flags = (MATCH_APPEND if append else 0) | {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path]
+    if plane is None:
+        _check(lib().yume_pdq_amd_match(
+            C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
+            int(threshold), C.c_void_p(hits.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
+            index_base, flags, _stream_ptr(stream)), "match")
+    else:
+        _check(lib().yume_pdq_amd_match_with_plane(
+            C.c_void_p(db.data_ptr()), C.c_void_p(plane.data_ptr()), db.shape[0],
+            C.c_void_p(queries.data_ptr()), queries.shape[0], int(threshold), C.c_void_p(hits.data_ptr()),
+            capacity, C.c_void_p(count.data_ptr()), index_base, flags, _stream_ptr(stream)), "match")
     if not sync:
         return hits, count
     total = int(count.item())
