@@ -114,11 +114,12 @@ int yume_pdq_amd_match(const uint8_t *d_db, uint64_t n_rows, const uint8_t *d_qu
                        uint32_t flags, void *stream);
 
 /*
- * Prefix plane: a second, matcher-ordered copy of the first 16 bytes of every
- * database row (16-row tiles stored word-major, zero-padded to whole tiles),
- * built once per database (and again after rows change).  With it the
- * matrix-core matcher reads 256 contiguous bytes per 16-row tile instead of
- * 16 scattered half-rows: half the bytes and no partially used cache lines.
+ * Prefix plane: the first 16 bytes of every database row, packed (row r at
+ * byte 16 r, zero-padded to a whole number of 16-row tiles), built once per
+ * database (and again after rows change).  Every pair is decided on these 128
+ * bits except the rare ones whose prefix distance is within the threshold,
+ * which read the rest of their row; so both matchers stream 16 bytes per row
+ * instead of 32, in whole cache lines.
  * No reference counterpart (a device-side index of the database; the
  * reference's VulkanMatcher likewise keeps its own device copy,
  * src/matching/vulkan.rs:40-64).

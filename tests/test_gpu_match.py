@@ -61,14 +61,16 @@ def test_planted_controls(gpu, oracle, dev, nq):
         assert first == (int(pos), i)
 
 
-THRESHOLD_CASES = [(t, p) for t in (0, 1, 31, 63, 64, 128, 200, 255, 256) for p in ("valu", "mfma", "mfma_plane")
+THRESHOLD_CASES = [(t, p) for t in (0, 1, 31, 63, 64, 128, 200, 255, 256)
+                   for p in ("valu", "mfma", "valu_plane", "mfma_plane")
                    if not (p.startswith("mfma") and t >= 64)]
 
 
 def run_path(gpu, d_db, d_q, threshold, path, **kw):
-    """path: valu / mfma (raw rows) / mfma_plane (matrix cores over the prefix plane)."""
-    if path == "mfma_plane":
-        return gpu.match(d_db, d_q, threshold, path="mfma", plane=gpu.build_prefix_plane(d_db), **kw)
+    """path: valu / mfma over the raw rows, valu_plane / mfma_plane over the
+    prefix plane."""
+    if path.endswith("_plane"):
+        return gpu.match(d_db, d_q, threshold, path=path[:-6], plane=gpu.build_prefix_plane(d_db), **kw)
     return gpu.match(d_db, d_q, threshold, path=path, **kw)
 
 
@@ -105,19 +107,17 @@ def test_mfma_query_counts(gpu, oracle, dev, nq, plane):
 
 
 def test_prefix_plane_layout(gpu, dev):
-    """plane[tile * 64 + kq * 16 + rn] = word kq of row 16 tile + rn, zero past
-    the last row (checked on the host against numpy)."""
+    """The plane holds the first 16 bytes of every row, packed, zero-padded to
+    whole 16-row tiles (checked on the host against numpy)."""
     import torch
     for n in (1, 15, 16, 17, 1000):
         db = synthetic.random_db(n, seed=n)
         say(f"[match] prefix plane layout n={n}")
-        plane = gpu.build_prefix_plane(torch.from_numpy(db).to(dev)).cpu().numpy().view(np.uint32)
-        tiles = (n + 15) // 16
-        assert plane.size == tiles * 64
-        words = np.zeros((tiles * 16, 8), np.uint32)
-        words[:n] = db.view(np.uint32).reshape(n, 8)
-        exp = words[:, :4].reshape(tiles, 16, 4).transpose(0, 2, 1).reshape(-1)
-        assert np.array_equal(plane, exp)
+        plane = gpu.build_prefix_plane(torch.from_numpy(db).to(dev)).cpu().numpy()
+        padded = (n + 15) // 16 * 16
+        exp = np.zeros((padded, 16), np.uint8)
+        exp[:n] = db[:, :16]
+        assert np.array_equal(plane.reshape(-1, 16), exp)
 
 
 def test_empty_and_tiny(gpu, dev):
@@ -165,15 +165,20 @@ def test_index_base_and_append(gpu, oracle, dev):
 
 @pytest.mark.parametrize("n_rows,nq", [(1_000_000, 8), (1_000_000, 128), (10_000_000, 8)])
 def test_scale_exact_set(gpu, dev, n_rows, nq):
-    """Config-sized DBs with planted near-duplicates: exact set vs numpy brute force."""
+    """Config-sized DBs with planted near-duplicates: exact set vs numpy brute
+    force, over the raw rows and over the prefix plane."""
     import torch
     db = synthetic.random_db(n_rows, seed=7)
     q = synthetic.random_db(nq, seed=8)
     synthetic.plant(db, q, max(1000, n_rows // 200), seed=12)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
     say(f"[match] scale n_rows={n_rows} nq={nq}")
-    total, hits = gpu.match(torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 31,
-                            capacity=1 << 20)
+    total, hits = gpu.match(d_db, d_q, 31, capacity=1 << 20)
     exp = synthetic.brute_force_hits(db, q, 31)
+    assert total == exp.shape[0]
+    assert np.array_equal(as_triples(hits), exp)
+    say(f"[match] scale n_rows={n_rows} nq={nq} prefix plane")
+    total, hits = gpu.match(d_db, d_q, 31, capacity=1 << 20, plane=gpu.build_prefix_plane(d_db))
     assert total == exp.shape[0]
     assert np.array_equal(as_triples(hits), exp)
 

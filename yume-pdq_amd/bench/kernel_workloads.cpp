@@ -5,7 +5,7 @@
 //                   scale for the FETCH_SIZE / WRITE_SIZE counters)
 //   match_x8        exhaustive match of one image's 8 dihedral hashes
 //   match_x1024     exhaustive match of 128 images' 1024 dihedral hashes
-//   match_plane_x1024  the same over the database's prefix plane
+//   match_plane_x8, match_plane_x1024  the same over the database's prefix plane
 //   build_plane     building the prefix plane (not in the default set)
 //   hash_u8         fused hash of --batch u8 planes
 // The database holds 50,000 planted near-duplicates (U{0..40} distinct bit
@@ -78,7 +78,7 @@ int main(int argc, char** argv) {
   uint64_t rows = 10'000'000;
   int steps = 20, warmup = 3;
   uint32_t batch = 128;
-  std::string cases = "copy_db,match_x8,match_x1024,match_plane_x1024,hash_u8";
+  std::string cases = "copy_db,match_x8,match_x1024,match_plane_x8,match_plane_x1024,hash_u8";
   for (int i = 1; i + 1 < argc; i += 2) {
     const std::string a = argv[i];
     if (a == "--rows") rows = std::strtoull(argv[i + 1], nullptr, 10);
@@ -178,6 +178,11 @@ int main(int argc, char** argv) {
   api_ok(yume_pdq_amd_build_prefix_plane(d_db, rows, d_plane, s), "build_prefix_plane");
   all.push_back({"build_plane", [&] {
                    api_ok(yume_pdq_amd_build_prefix_plane(d_db, rows, d_plane, s), "build_prefix_plane");
+                 }});
+  all.push_back({"match_plane_x8", [&] {
+                   api_ok(yume_pdq_amd_match_with_plane(d_db, d_plane, rows, d_hashes, 8, 31, d_hits, cap, d_count, 0,
+                                                        0, s),
+                          "match_with_plane");
                  }});
   all.push_back({"match_plane_x1024", [&] {
                    api_ok(yume_pdq_amd_match_with_plane(d_db, d_plane, rows, d_hashes, 1024, 31, d_hits, cap, d_count,

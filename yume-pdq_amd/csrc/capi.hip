@@ -279,6 +279,9 @@ static int match_impl(const uint8_t* d_db, const void* d_plane, uint64_t n_rows,
   if (mfma)
     return from_hip(ypdq::launch_match_mfma(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
                                             d_count, index_base, st));
+  if (d_plane)
+    return from_hip(ypdq::launch_match_plane(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
+                                             d_count, index_base, st));
   return from_hip(
       ypdq::launch_match(d_db, n_rows, d_queries, n_queries, threshold, hits, capacity, d_count, index_base, st));
 }

@@ -64,6 +64,15 @@ hipError_t launch_match_tuned(const MatchTuning& t, const void* db, uint64_t n_r
 hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
                         MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
                         hipStream_t stream);
+// Vector-ALU scan over the prefix plane (16 bytes per row streamed).
+MatchTuning default_plane_tuning(uint64_t n_rows, uint32_t n_queries);
+hipError_t launch_match_plane_tuned(const MatchTuning& t, const void* db, const void* plane, uint64_t n_rows,
+                                    const void* queries, uint32_t n_queries, uint32_t threshold, MatchHit* hits,
+                                    uint64_t capacity, unsigned long long* counter, uint64_t index_base,
+                                    hipStream_t stream);
+hipError_t launch_match_plane(const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                              unsigned long long* counter, uint64_t index_base, hipStream_t stream);
 // Matrix-core prefix filter + exact re-check (threshold < 64, >= 32 queries).
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold);
 // plane: the database's prefix plane (launch_build_prefix_plane), or null to

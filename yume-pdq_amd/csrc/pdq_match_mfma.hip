@@ -130,16 +130,16 @@ __device__ __forceinline__ void resolve_tile(const Expanded (&qa)[QT], const Exp
   }
 }
 
-// Where a wave's row words come from.  Raw rows: lane (rn, kq) reads word kq
-// of row r at db[r * 8 + kq] -- 16 of every 32 bytes, scattered over 4 cache
-// lines per tile.  Prefix plane: the same words stored tile-major,
-// plane[tile * 64 + kq * 16 + rn], so a tile is 256 contiguous bytes and a
-// launch reads half the bytes.
+// Where a wave's row words come from.  Lane (rn, kq) needs word kq of row
+// 16 * tile + rn.  Raw rows: db[r * 8 + kq] -- 16 of every 32 bytes,
+// scattered over 4 cache lines per tile.  Prefix plane (the first 16 bytes of
+// every row, packed): plane[r * 4 + kq], so a tile is 256 contiguous bytes and
+// a launch reads half the bytes.
 template <bool kPlane>
 struct RowSource {
   const uint32_t* __restrict__ p;
   __device__ __forceinline__ RowSource(const uint32_t* db, const uint32_t* plane, uint64_t row0, int lane)
-      : p(kPlane ? plane + (row0 / 16) * 64 + lane : db + (row0 + (lane & 15)) * 8 + (lane >> 4)) {}
+      : p(kPlane ? plane + (row0 + (lane & 15)) * 4 + (lane >> 4) : db + (row0 + (lane & 15)) * 8 + (lane >> 4)) {}
   __device__ __forceinline__ uint32_t operator()(uint32_t tile) const {
     return p[(size_t)tile * (kPlane ? 64 : 128)];
   }
@@ -284,27 +284,24 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
   return hipGetLastError();
 }
 
-// Prefix plane: for tile T (rows 16T .. 16T+15) and lane L = kq * 16 + rn,
-// plane[T * 64 + L] = word kq of row 16T + rn (0 past the last row).
-__global__ void build_prefix_plane_kernel(const uint32_t* __restrict__ db, uint64_t n_rows,
-                                          uint32_t* __restrict__ plane, uint64_t n_words) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_words) return;
-  const uint64_t tile = i >> 6;
-  const uint32_t lane = (uint32_t)(i & 63);
-  const uint64_t row = tile * 16 + (lane & 15);
-  plane[i] = row < n_rows ? db[row * 8 + (lane >> 4)] : 0u;
+// Prefix plane: plane[r * 4 + k] = word k of row r (the row's first 16
+// bytes), zero for the padding rows up to a whole 16-row tile.
+__global__ void build_prefix_plane_kernel(const uint4* __restrict__ db, uint64_t n_rows, uint4* __restrict__ plane,
+                                          uint64_t n_padded) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_padded) return;
+  plane[r] = r < n_rows ? db[2 * r] : make_uint4(0, 0, 0, 0);
 }
 
 uint64_t prefix_plane_bytes(uint64_t n_rows) { return (n_rows + 15) / 16 * 256; }
 
 hipError_t launch_build_prefix_plane(const void* db, uint64_t n_rows, void* plane, hipStream_t stream) {
-  const uint64_t n_words = (n_rows + 15) / 16 * 64;
-  if (n_words == 0) return hipSuccess;
-  const uint64_t blocks = (n_words + 255) / 256;
+  const uint64_t n_padded = (n_rows + 15) / 16 * 16;
+  if (n_padded == 0) return hipSuccess;
+  const uint64_t blocks = (n_padded + 255) / 256;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(build_prefix_plane_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
-                     static_cast<const uint32_t*>(db), n_rows, static_cast<uint32_t*>(plane), n_words);
+                     static_cast<const uint4*>(db), n_rows, static_cast<uint4*>(plane), n_padded);
   return hipGetLastError();
 }
 
