@@ -190,6 +190,89 @@ int yume_pdq_amd_stage_transform(float *d_pdqf, uint32_t n, int op, void *stream
 int yume_pdq_amd_stage_luma(const uint8_t *d_pixels, int channels, uint64_t n_pixels, float r_coeff,
                             float g_coeff, float b_coeff, float *d_out, void *stream);
 
+/*
+ * Batching layer.  No reference counterpart: the reference hashes one image
+ * per call and matches one batch of needles against one database
+ * (src/matching/mod.rs, src/matching/vulkan.rs); these entry points batch
+ * both over many images and many database shards.
+ *
+ * Shard set: database shards of any size resident in HBM, each with its
+ * prefix plane and the global index of its first row.  Rows given on the host
+ * are copied in (owned); rows given on the device are referenced (they must
+ * outlive the set, 16-byte aligned).  Do not add shards while batches that use
+ * the set are running.
+ */
+typedef struct YumePdqAmdShards YumePdqAmdShards;
+int yume_pdq_amd_shards_create(YumePdqAmdShards **out);
+void yume_pdq_amd_shards_destroy(YumePdqAmdShards *shards);
+int yume_pdq_amd_shards_add(YumePdqAmdShards *shards, const uint8_t *rows, uint64_t n_rows,
+                            uint64_t index_base, int rows_on_device, uint32_t *shard_id);
+/* A raw file of N x 32-byte rows (the hash byte order), split into shards of
+ * rows_per_shard rows; global indices start at index_base. */
+int yume_pdq_amd_shards_load_raw(YumePdqAmdShards *shards, const char *path, uint64_t rows_per_shard,
+                                 uint64_t index_base, uint32_t *n_added);
+/* In place, on `stream`: row d_local_rows[i] of shard `shard` becomes
+ * d_new_rows[i] (32 bytes each, prefix plane updated too); indices must be
+ * distinct; out-of-range indices are skipped and counted into *d_bad. */
+int yume_pdq_amd_shards_replace(YumePdqAmdShards *shards, uint32_t shard, const uint64_t *d_local_rows,
+                                const uint8_t *d_new_rows, uint32_t n, unsigned long long *d_bad,
+                                void *stream);
+int yume_pdq_amd_shards_info(const YumePdqAmdShards *shards, uint32_t *n_shards, uint64_t *total_rows,
+                             uint64_t *device_bytes);
+/* yume_pdq_amd_match over every shard of the set in one persistent launch;
+ * hit indices are global (shard index_base + row). */
+int yume_pdq_amd_shards_match(const YumePdqAmdShards *shards, const uint8_t *d_queries,
+                              uint32_t n_queries, uint32_t threshold, YumePdqAmdHit *d_hits,
+                              uint64_t capacity, unsigned long long *d_count, uint32_t flags,
+                              void *stream);
+
+/*
+ * Batch context: device buffers for up to max_images planes and their
+ * outputs, and one HIP graph per matcher choice holding the hash kernel, the
+ * hit-counter reset and a persistent matcher over every shard.  Both kernels
+ * read the batch size from device memory, so a batch of any size replays the
+ * same graph.  run() enqueues on the context's own stream (it first waits
+ * for the context's previous batch); read the outputs after synchronising
+ * that stream.  Contexts over one shard set may run concurrently on their
+ * streams.
+ */
+typedef struct YumePdqAmdBatcher YumePdqAmdBatcher;
+typedef struct YumePdqAmdBatchBuffers {
+  void *d_input;              /* max_images planes, input_stride_bytes apart */
+  uint64_t input_stride_bytes;
+  uint8_t *d_hashes;          /* max_images x 8 x 32 */
+  float *d_quality;           /* max_images x 8 */
+  float *d_threshold;         /* max_images x 8 */
+  YumePdqAmdHit *d_hits;      /* hit_capacity records */
+  uint64_t hit_capacity;
+  unsigned long long *d_count;
+  void *stream;               /* hipStream_t of the context */
+  uint32_t max_images;
+  uint64_t device_bytes;      /* HBM held by the context */
+} YumePdqAmdBatchBuffers;
+typedef struct YumePdqAmdTuneRecord {
+  uint32_t n_queries;
+  uint32_t path;              /* YUME_PDQ_AMD_MATCH_VALU or _MFMA */
+  uint32_t query_tiles;       /* 16-query tiles per wave (matrix cores) */
+  float ms;
+} YumePdqAmdTuneRecord;
+int yume_pdq_amd_batcher_create(YumePdqAmdShards *shards, uint32_t max_images, int input_kind,
+                                uint32_t dihedral_mode, uint32_t threshold, uint64_t hit_capacity,
+                                YumePdqAmdBatcher **out);
+void yume_pdq_amd_batcher_destroy(YumePdqAmdBatcher *b);
+int yume_pdq_amd_batcher_buffers(const YumePdqAmdBatcher *b, YumePdqAmdBatchBuffers *out);
+/* Hash input planes [0, n_images) and match their 8 n_images hashes against
+ * every shard (use_graph = 0 enqueues the same kernels without a graph). */
+int yume_pdq_amd_batcher_run(YumePdqAmdBatcher *b, uint32_t n_images, int use_graph);
+/* Time the vector-ALU matcher and the matrix-core matcher at 2/4/8/16 query
+ * tiles per wave for query counts 8, 16, 32, ... 8 max_images on the set's
+ * shards, and use the fastest per bucket from then on. */
+int yume_pdq_amd_batcher_autotune(YumePdqAmdBatcher *b, int iterations);
+int yume_pdq_amd_batcher_tune_records(const YumePdqAmdBatcher *b, YumePdqAmdTuneRecord *out,
+                                      uint32_t cap, uint32_t *n);
+int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher *b, uint32_t n_queries, uint32_t *path,
+                                uint32_t *query_tiles);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

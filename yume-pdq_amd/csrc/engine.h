@@ -1,0 +1,141 @@
+// Batching layer (host side): database shards in HBM and batch contexts that
+// replay one captured hash -> match graph per batch.  See engine.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "pdq_device.h"
+
+namespace ypdq {
+
+// Which matcher a batch uses: the vector-ALU scan, or the matrix-core filter
+// with qt 16-query tiles per wave.
+struct MatcherChoice {
+  bool mfma = false;
+  int qt = 8;
+};
+
+struct TuneRecord {
+  uint32_t n_queries;
+  MatcherChoice choice;
+  float ms;
+};
+
+class ShardSet {
+ public:
+  ShardSet() : cus_(0) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    cus_ = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) ? p.multiProcessorCount
+                                                                                             : 256;
+  }
+  ~ShardSet();
+  ShardSet(const ShardSet&) = delete;
+  ShardSet& operator=(const ShardSet&) = delete;
+
+  // rows: host (copied into HBM, owned) or device (referenced, must outlive
+  // the set; 16-byte aligned).  The prefix plane is built either way.
+  hipError_t add(const uint8_t* rows, uint64_t n_rows, uint64_t index_base, bool host, uint32_t* id);
+  // A raw file of N x 32-byte rows, split into shards of rows_per_shard.
+  hipError_t load_raw_file(const char* path, uint64_t rows_per_shard, uint64_t index_base, uint32_t* n_added);
+  // rows[d_local_rows[i]] = d_new_rows[i] (and the prefix plane), on `stream`;
+  // out-of-range indices are counted in *d_bad and skipped.
+  hipError_t replace(uint32_t shard, const uint64_t* d_local_rows, const uint8_t* d_new_rows, uint32_t n,
+                     unsigned long long* d_bad, hipStream_t stream);
+
+  // One persistent launch over every shard.  d_nq: device query count (or
+  // null for nq_cap).
+  hipError_t launch_match(const MatcherChoice& c, const void* d_queries, const uint32_t* d_nq, uint32_t nq_cap,
+                          uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                          hipStream_t stream) const;
+
+  uint32_t count() const { return (uint32_t)shards_.size(); }
+  uint64_t total_rows() const;
+  uint64_t device_bytes() const;
+  uint64_t version() const { return version_; }
+  const uint8_t* rows(uint32_t i) const { return i < shards_.size() ? shards_[i].rows : nullptr; }
+  uint64_t n_rows(uint32_t i) const { return i < shards_.size() ? shards_[i].n_rows : 0; }
+  uint64_t index_base(uint32_t i) const { return i < shards_.size() ? shards_[i].index_base : 0; }
+
+ private:
+  struct Shard {
+    uint8_t* rows = nullptr;
+    uint8_t* plane = nullptr;
+    uint64_t n_rows = 0;
+    uint64_t index_base = 0;
+    bool owns_rows = false;
+  };
+  hipError_t rebuild_views_locked();
+
+  int cus_;
+  std::mutex mu_;
+  std::vector<Shard> shards_;
+  ShardView* d_views_ = nullptr;
+  uint64_t total_chunks_ = 0, total_rblocks_ = 0;
+  uint64_t version_ = 0;
+};
+
+class BatchContext {
+ public:
+  BatchContext() = default;
+  ~BatchContext();
+  BatchContext(const BatchContext&) = delete;
+  BatchContext& operator=(const BatchContext&) = delete;
+
+  hipError_t init(std::shared_ptr<ShardSet> shards, uint32_t max_images, int input_kind, uint32_t mode,
+                  uint32_t threshold, uint64_t hit_capacity);
+  // Hash input[0 .. n_images) and match the 8n hashes against every shard,
+  // enqueued on stream() (a graph replay unless use_graph is false).  Waits
+  // for this context's previous batch first.
+  hipError_t run(uint32_t n_images, bool use_graph);
+  hipError_t autotune(int iterations, std::vector<TuneRecord>* records);
+  MatcherChoice choice_for(uint32_t n_queries) const;
+
+  hipStream_t stream() const { return stream_; }
+  void* input() const { return d_input_; }
+  uint64_t plane_stride() const { return plane_stride_; }
+  uint8_t* hashes() const { return d_hashes_; }
+  float* quality() const { return d_quality_; }
+  float* threshold_out() const { return d_threshold_; }
+  void* hits() const { return d_hits_; }
+  unsigned long long* count() const { return d_count_; }
+  uint32_t max_images() const { return max_images_; }
+  uint64_t hit_capacity() const { return hit_capacity_; }
+  uint64_t device_bytes() const;
+
+ private:
+  struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t version = 0;
+  };
+  hipError_t enqueue_body(const MatcherChoice& c, hipStream_t s);
+  hipError_t graph_for(const MatcherChoice& c, hipGraphExec_t* exec);
+
+  std::shared_ptr<ShardSet> shards_;
+  uint32_t max_images_ = 0;
+  int input_kind_ = kInputU8;
+  uint32_t mode_ = kDihedralExact;
+  uint32_t threshold_ = 31;
+  uint64_t hit_capacity_ = 0;
+  uint64_t plane_stride_ = 0;
+  hipStream_t stream_ = nullptr;
+  void* d_input_ = nullptr;
+  uint8_t* d_hashes_ = nullptr;
+  float* d_quality_ = nullptr;
+  float* d_threshold_ = nullptr;
+  void* d_hits_ = nullptr;
+  unsigned long long* d_count_ = nullptr;
+  uint32_t* d_sizes_ = nullptr;  // device {n_images, n_queries}
+  uint32_t* h_sizes_ = nullptr;  // pinned staging of the same
+  std::map<uint32_t, MatcherChoice> choices_;  // first query count of a bucket -> choice
+  std::map<int, Graph> graphs_;
+};
+
+}  // namespace ypdq

@@ -1,0 +1,391 @@
+// Batching layer: database shards resident in HBM, and batch contexts that
+// run "hash a batch of images -> match its 8n dihedral hashes against every
+// shard" as one HIP-graph launch.
+//
+//   ShardSet      any number of shards of any size (rows + prefix plane in
+//                 HBM, global indices from each shard's index_base), a raw
+//                 N x 32-byte file loader, in-place row replacement.  The
+//                 reference keeps its database as an in-memory &[[u8; 32]]
+//                 (src/matching/vulkan.rs:40-64) and replaces entries by
+//                 substitution (examples/end_to_end.rs:52-53).
+//   BatchContext  fixed device buffers for up to max_images images, and one
+//                 captured graph per matcher choice: the hash kernel (image
+//                 count read from device memory), the hit-counter reset (a
+//                 kernel, not a memset), and a persistent matcher walking the
+//                 work items of every shard (query count read from device
+//                 memory).  A batch of any size up to max_images replays a
+//                 graph after one 8-byte upload of its size.
+//   Autotuner     per query-count bucket, times the vector-ALU kernel and the
+//                 matrix-core kernel at 2/4/8/16 query tiles per wave on the
+//                 context's own shards and keeps the fastest.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "engine.h"
+
+namespace ypdq {
+
+namespace {
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// In-place replacement: rows[idx[i]] = new_rows[i], and the same row's plane
+// entry.  One thread per replaced row.
+__global__ void replace_rows_kernel(uint4* __restrict__ rows, uint4* __restrict__ plane, uint64_t n_rows,
+                                    const uint64_t* __restrict__ idx, const uint4* __restrict__ new_rows, uint32_t n,
+                                    unsigned long long* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t r = idx[i];
+  if (r >= n_rows) {
+    atomicAdd(bad, 1ull);
+    return;
+  }
+  const uint4 lo = new_rows[2 * i], hi = new_rows[2 * i + 1];
+  rows[2 * r] = lo;
+  rows[2 * r + 1] = hi;
+  if (plane) plane[r] = lo;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// ShardSet
+// ---------------------------------------------------------------------------
+ShardSet::~ShardSet() {
+  for (auto& s : shards_) {
+    if (s.owns_rows && s.rows) (void)hipFree(s.rows);
+    if (s.plane) (void)hipFree(s.plane);
+  }
+  if (d_views_) (void)hipFree(d_views_);
+}
+
+hipError_t ShardSet::add(const uint8_t* rows, uint64_t n_rows, uint64_t index_base, bool host, uint32_t* id) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (n_rows == 0) return hipErrorInvalidValue;
+  Shard s;
+  s.n_rows = n_rows;
+  s.index_base = index_base;
+  hipError_t e;
+  if (host) {
+    if ((e = hipMalloc(&s.rows, n_rows * 32)) != hipSuccess) return e;
+    s.owns_rows = true;
+    if ((e = hipMemcpy(s.rows, rows, n_rows * 32, hipMemcpyHostToDevice)) != hipSuccess) {
+      (void)hipFree(s.rows);
+      return e;
+    }
+  } else {
+    if ((uintptr_t)rows % 16) return hipErrorInvalidValue;
+    s.rows = const_cast<uint8_t*>(rows);
+  }
+  if ((e = hipMalloc(&s.plane, prefix_plane_bytes(n_rows))) != hipSuccess ||
+      (e = launch_build_prefix_plane(s.rows, n_rows, s.plane, nullptr)) != hipSuccess ||
+      (e = hipDeviceSynchronize()) != hipSuccess) {
+    if (s.owns_rows) (void)hipFree(s.rows);
+    if (s.plane) (void)hipFree(s.plane);
+    return e;
+  }
+  shards_.push_back(s);
+  if (id) *id = (uint32_t)(shards_.size() - 1);
+  version_++;
+  return rebuild_views_locked();
+}
+
+hipError_t ShardSet::load_raw_file(const char* path, uint64_t rows_per_shard, uint64_t index_base, uint32_t* n_added) {
+  if (!path || rows_per_shard == 0) return hipErrorInvalidValue;
+  struct stat st;
+  if (stat(path, &st) != 0 || st.st_size % 32) return hipErrorInvalidValue;
+  FILE* f = fopen(path, "rb");
+  if (!f) return hipErrorInvalidValue;
+  const uint64_t total = (uint64_t)st.st_size / 32;
+  // staged through a pinned buffer, one shard at a time
+  const uint64_t chunk = std::min<uint64_t>(rows_per_shard, total ? total : 1);
+  uint8_t* pinned = nullptr;
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&pinned), chunk * 32, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    fclose(f);
+    return e;
+  }
+  uint32_t added = 0;
+  for (uint64_t first = 0; first < total && e == hipSuccess; first += rows_per_shard) {
+    const uint64_t n = std::min<uint64_t>(rows_per_shard, total - first);
+    if (fread(pinned, 32, n, f) != n) {
+      e = hipErrorInvalidValue;
+      break;
+    }
+    e = add(pinned, n, index_base + first, true, nullptr);
+    if (e == hipSuccess) added++;
+  }
+  (void)hipHostFree(pinned);
+  fclose(f);
+  if (n_added) *n_added = added;
+  return e;
+}
+
+hipError_t ShardSet::replace(uint32_t shard, const uint64_t* d_local_rows, const uint8_t* d_new_rows, uint32_t n,
+                             unsigned long long* d_bad, hipStream_t stream) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (shard >= shards_.size()) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  if (!d_local_rows || !d_new_rows || !d_bad || (uintptr_t)d_new_rows % 16) return hipErrorInvalidValue;
+  const Shard& s = shards_[shard];
+  hipLaunchKernelGGL(replace_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, reinterpret_cast<uint4*>(s.rows),
+                     reinterpret_cast<uint4*>(s.plane), s.n_rows, d_local_rows,
+                     reinterpret_cast<const uint4*>(d_new_rows), n, d_bad);
+  return hipGetLastError();
+}
+
+hipError_t ShardSet::rebuild_views_locked() {
+  std::vector<ShardView> v(shards_.size());
+  uint64_t chunk = 0, rblock = 0;
+  const MatchTuning vt = default_plane_tuning(0, 0);
+  for (size_t i = 0; i < shards_.size(); i++) {
+    v[i].rows = reinterpret_cast<const uint4*>(shards_[i].rows);
+    v[i].plane = reinterpret_cast<const uint4*>(shards_[i].plane);
+    v[i].n_rows = shards_[i].n_rows;
+    v[i].index_base = shards_[i].index_base;
+    v[i].chunk_begin = chunk;
+    v[i].rblock_begin = rblock;
+    chunk += mfma_chunks(shards_[i].n_rows);
+    rblock += valu_rblocks(vt, shards_[i].n_rows);
+  }
+  total_chunks_ = chunk;
+  total_rblocks_ = rblock;
+  if (d_views_) (void)hipFree(d_views_);
+  d_views_ = nullptr;
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(&d_views_, v.size() * sizeof(ShardView));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(d_views_, v.data(), v.size() * sizeof(ShardView), hipMemcpyHostToDevice);
+}
+
+uint64_t ShardSet::total_rows() const {
+  uint64_t t = 0;
+  for (auto& s : shards_) t += s.n_rows;
+  return t;
+}
+
+uint64_t ShardSet::device_bytes() const {
+  uint64_t t = 0;
+  for (auto& s : shards_) t += (s.owns_rows ? s.n_rows * 32 : 0) + prefix_plane_bytes(s.n_rows);
+  return t;
+}
+
+hipError_t ShardSet::launch_match(const MatcherChoice& c, const void* d_queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                  uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                                  hipStream_t stream) const {
+  if (shards_.empty()) return hipSuccess;
+  const int cus = cus_;
+  if (c.mfma) {
+    // resident blocks: 4 waves per SIMD at 2..8 query tiles, 2 at 16
+    uint64_t grid = (uint64_t)cus * (c.qt >= 16 ? 2 : 4);
+    const uint64_t groups = (nq_cap + c.qt * 16 - 1) / (c.qt * 16);
+    const uint64_t items = total_chunks_ * groups;
+    if (grid > items) grid = (items + 7) / 8 * 8;
+    grid = std::max<uint64_t>(8, grid / 8 * 8);
+    return launch_match_mfma_shards(c.qt, d_views_, (uint32_t)shards_.size(), total_chunks_, true, d_queries, d_nq,
+                                    nq_cap, 0, threshold, hits, capacity, counter, (uint32_t)grid, stream);
+  }
+  const MatchTuning vt = default_plane_tuning(0, 0);
+  uint64_t grid = (uint64_t)cus * 16;
+  if (grid > total_rblocks_) grid = total_rblocks_;
+  return launch_match_valu_shards(vt, d_views_, (uint32_t)shards_.size(), total_rblocks_, true, d_queries, d_nq,
+                                  nq_cap, 0xffffffffu, threshold, hits, capacity, counter,
+                                  (uint32_t)std::max<uint64_t>(1, grid), stream);
+}
+
+// ---------------------------------------------------------------------------
+// BatchContext
+// ---------------------------------------------------------------------------
+BatchContext::~BatchContext() {
+  for (auto& kv : graphs_) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+  }
+  void* bufs[] = {d_input_, d_hashes_, d_quality_, d_threshold_, d_hits_, d_count_, d_sizes_};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (h_sizes_) (void)hipHostFree(h_sizes_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipError_t BatchContext::init(std::shared_ptr<ShardSet> shards, uint32_t max_images, int input_kind, uint32_t mode,
+                              uint32_t threshold, uint64_t hit_capacity) {
+  if (max_images == 0 || (input_kind != kInputF32 && input_kind != kInputU8) || mode > 1 || threshold > 256 ||
+      hit_capacity == 0)
+    return hipErrorInvalidValue;
+  shards_ = std::move(shards);
+  max_images_ = max_images;
+  input_kind_ = input_kind;
+  mode_ = mode;
+  threshold_ = threshold;
+  hit_capacity_ = hit_capacity;
+  plane_stride_ = 512ull * 512 * (input_kind == kInputF32 ? 4 : 1);
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_input_, plane_stride_ * max_images)) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_hashes_, (size_t)max_images * 256)) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_quality_, (size_t)max_images * 32)) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_threshold_, (size_t)max_images * 32)) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_hits_, hit_capacity * sizeof(MatchHit))) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_count_, sizeof(unsigned long long))) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_sizes_, 2 * sizeof(uint32_t))) != hipSuccess) return e;
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_sizes_), 2 * sizeof(uint32_t), hipHostMallocDefault)) !=
+      hipSuccess)
+    return e;
+  // default choices before any autotuning: vector ALU below 32 queries, eight
+  // query tiles per wave on the matrix cores above
+  choices_.clear();
+  return hipSuccess;
+}
+
+uint64_t BatchContext::device_bytes() const {
+  return plane_stride_ * max_images_ + (uint64_t)max_images_ * (256 + 64) + hit_capacity_ * sizeof(MatchHit) + 16;
+}
+
+MatcherChoice BatchContext::choice_for(uint32_t nq) const {
+  auto it = choices_.upper_bound(nq);
+  if (it != choices_.begin()) {
+    --it;
+    return it->second;
+  }
+  MatcherChoice c;
+  c.mfma = threshold_ < 64 && nq >= 32;
+  c.qt = 8;
+  return c;
+}
+
+static int choice_key(const MatcherChoice& c) { return c.mfma ? c.qt : 0; }
+
+hipError_t BatchContext::enqueue_body(const MatcherChoice& c, hipStream_t s) {
+  HashOutputs o;
+  memset(&o, 0, sizeof o);
+  o.hashes = d_hashes_;
+  o.quality = d_quality_;
+  o.threshold = d_threshold_;
+  hipError_t e = launch_hash_device_count(d_input_, input_kind_, plane_stride_, max_images_, d_sizes_, mode_, o, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, d_count_, 0ull);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return shards_->launch_match(c, d_hashes_, d_sizes_ + 1, max_images_ * 8, threshold_,
+                               reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, s);
+}
+
+hipError_t BatchContext::graph_for(const MatcherChoice& c, hipGraphExec_t* exec) {
+  const int key = choice_key(c);
+  auto it = graphs_.find(key);
+  if (it != graphs_.end() && it->second.version == shards_->version()) {
+    *exec = it->second.exec;
+    return hipSuccess;
+  }
+  if (it != graphs_.end()) {
+    (void)hipGraphExecDestroy(it->second.exec);
+    (void)hipGraphDestroy(it->second.graph);
+    graphs_.erase(it);
+  }
+  Graph g;
+  hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t body = enqueue_body(c, stream_);
+  e = hipStreamEndCapture(stream_, &g.graph);
+  if (body != hipSuccess) {
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    return body;
+  }
+  if (e != hipSuccess) return e;
+  if ((e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0)) != hipSuccess) {
+    (void)hipGraphDestroy(g.graph);
+    return e;
+  }
+  g.version = shards_->version();
+  graphs_[key] = g;
+  *exec = g.exec;
+  return hipSuccess;
+}
+
+hipError_t BatchContext::run(uint32_t n_images, bool use_graph) {
+  if (n_images > max_images_) return hipErrorInvalidValue;
+  const MatcherChoice c = choice_for(n_images * 8);
+  // the batch size goes to device memory through pinned staging; the previous
+  // batch on this stream has consumed its copy by the time this one is read,
+  // but its upload may still be in flight, so wait for it before rewriting
+  hipError_t e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return e;
+  h_sizes_[0] = n_images;
+  h_sizes_[1] = n_images * 8;
+  if ((e = hipMemcpyAsync(d_sizes_, h_sizes_, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream_)) != hipSuccess)
+    return e;
+  if (!use_graph) return enqueue_body(c, stream_);
+  hipGraphExec_t exec = nullptr;
+  if ((e = graph_for(c, &exec)) != hipSuccess) return e;
+  return hipGraphLaunch(exec, stream_);
+}
+
+hipError_t BatchContext::autotune(int iterations, std::vector<TuneRecord>* records) {
+  if (iterations < 1) iterations = 1;
+  choices_.clear();
+  hipEvent_t a, b;
+  hipError_t e;
+  if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+  if ((e = hipEventCreate(&b)) != hipSuccess) return e;
+  // queries: whatever the hash buffer holds; timing does not depend on values
+  std::vector<uint32_t> buckets;
+  for (uint32_t nq = 8; nq <= max_images_ * 8; nq *= 2) buckets.push_back(nq);
+  if (buckets.empty() || buckets.back() != max_images_ * 8) buckets.push_back(max_images_ * 8);
+  for (uint32_t nq : buckets) {
+    std::vector<MatcherChoice> cands;
+    MatcherChoice v;
+    v.mfma = false;
+    cands.push_back(v);
+    if (threshold_ < 64)
+      for (int qt : {2, 4, 8, 16}) {
+        if (qt > 2 && (uint32_t)(qt / 2) * 16 >= nq) break;  // no more than one tile of padding
+        MatcherChoice m;
+        m.mfma = true;
+        m.qt = qt;
+        cands.push_back(m);
+      }
+    float best = 1e30f;
+    MatcherChoice best_c = cands[0];
+    for (const MatcherChoice& c : cands) {
+      h_sizes_[0] = nq / 8;
+      h_sizes_[1] = nq;
+      if ((e = hipMemcpyAsync(d_sizes_, h_sizes_, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream_)) !=
+          hipSuccess)
+        return e;
+      auto launch = [&]() -> hipError_t {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, stream_, d_count_, 0ull);
+        return shards_->launch_match(c, d_hashes_, d_sizes_ + 1, nq, threshold_,
+                                     reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, stream_);
+      };
+      if ((e = launch()) != hipSuccess) return e;  // warm-up
+      if ((e = hipEventRecord(a, stream_)) != hipSuccess) return e;
+      for (int i = 0; i < iterations; i++)
+        if ((e = launch()) != hipSuccess) return e;
+      if ((e = hipEventRecord(b, stream_)) != hipSuccess) return e;
+      if ((e = hipEventSynchronize(b)) != hipSuccess) return e;
+      float ms = 0;
+      if ((e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) return e;
+      ms /= iterations;
+      if (records) records->push_back(TuneRecord{nq, c, ms});
+      if (ms < best) {
+        best = ms;
+        best_c = c;
+      }
+    }
+    choices_[nq] = best_c;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return hipSuccess;
+}
+
+}  // namespace ypdq

@@ -47,6 +47,9 @@ __device__ __forceinline__ bool wave_all(bool b) { return __all(b ? 1 : 0) != 0;
 
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
                        const HashOutputs& out, hipStream_t stream);
+// Grid for n_max images, image count read from d_n (capped at n_max).
+hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                                    const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream);
 hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, uint32_t mode,
                             const HashOutputs& out, hipStream_t stream);
 // Shape of a matcher launch: rows per lane per iteration, threads per block,
@@ -64,6 +67,27 @@ hipError_t launch_match_tuned(const MatchTuning& t, const void* db, uint64_t n_r
 hipError_t launch_match(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries, uint32_t threshold,
                         MatchHit* hits, uint64_t capacity, unsigned long long* counter, uint64_t index_base,
                         hipStream_t stream);
+// A database shard as the persistent multi-shard matchers see it.  Work
+// items are numbered over all shards: MFMA row chunks from chunk_begin (a
+// multiple of 8), vector-ALU row blocks from rblock_begin.
+struct ShardView {
+  const uint4* rows;   // n_rows x 32 bytes
+  const uint4* plane;  // prefix plane (may be null for the raw-row kernels)
+  uint64_t n_rows;
+  uint64_t index_base;  // global index of row 0
+  uint64_t chunk_begin;
+  uint64_t rblock_begin;
+};
+uint64_t mfma_chunks(uint64_t n_rows);  // MFMA work-item chunks of one shard (multiple of 8)
+hipError_t launch_match_mfma_shards(int qt, const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
+                                    bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                    uint32_t min_nq, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                    unsigned long long* counter, uint32_t grid, hipStream_t stream);
+hipError_t launch_match_valu_shards(const MatchTuning& t, const ShardView* shards, uint32_t n_shards,
+                                    uint64_t total_rblocks, bool plane, const void* queries, const uint32_t* d_nq,
+                                    uint32_t nq_cap, uint32_t max_nq, uint32_t threshold, MatchHit* hits,
+                                    uint64_t capacity, unsigned long long* counter, uint32_t grid, hipStream_t stream);
+uint64_t valu_rblocks(const MatchTuning& t, uint64_t n_rows);  // VALU work items of one shard
 // Vector-ALU scan over the prefix plane (16 bytes per row streamed).
 MatchTuning default_plane_tuning(uint64_t n_rows, uint32_t n_queries);
 hipError_t launch_match_plane_tuned(const MatchTuning& t, const void* db, const void* plane, uint64_t n_rows,

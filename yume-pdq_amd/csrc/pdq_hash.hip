@@ -357,20 +357,8 @@ __device__ __forceinline__ bool plane_needs_literal(const uint8_t* __restrict__,
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) pdq_hash_kernel(const T* __restrict__ planes, uint64_t plane_stride,
-                                                        uint32_t n_images, uint32_t mode, HashOutputs out) {
-  __shared__ BlockLds L;
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  // padded DCT rows for stage 2: d2[j][m] = D_flat[j*127 + m] (m = 127 reads
-  // the next row's first coefficient, or the zero pad after row 15)
-  for (int e = threadIdx.x; e < 16 * 128; e += blockDim.x) {
-    const int j = e >> 7, m = e & 127;
-    L.d2[e] = kDct[j * 127 + m];
-  }
-  __syncthreads();
-  const uint32_t img = blockIdx.x * kWavesPerBlock + wv;
-  if (img >= n_images) return;
+__device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t plane_stride, uint32_t img, int lane,
+                                         int wv, uint32_t mode, const HashOutputs& out, BlockLds& L) {
   const T* plane = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(planes) + (size_t)img * plane_stride);
   float* buf1 = out.buf1 ? out.buf1 + (size_t)img * 128 * 128 : nullptr;
   WaveLds& W = L.w[wv];
@@ -390,6 +378,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) p
   }
   stage2_to_pdqf(L, W, lane, out, img);
   dihedral_tail(W, lane, mode, out, img, nullptr);
+}
+
+// One wave per image; the grid covers n_images.  With d_n set, the image
+// count is read from device memory (capped at n_images) and the waves past it
+// return at once, so one captured launch sized for the largest batch serves
+// batches of any size.
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) pdq_hash_kernel(const T* __restrict__ planes, uint64_t plane_stride,
+                                                        uint32_t n_images, const uint32_t* __restrict__ d_n,
+                                                        uint32_t mode, HashOutputs out) {
+  const uint32_t n = d_n ? min(*d_n, n_images) : n_images;
+  if (blockIdx.x * kWavesPerBlock >= n) return;
+  __shared__ BlockLds L;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  // padded DCT rows for stage 2: d2[j][m] = D_flat[j*127 + m] (m = 127 reads
+  // the next row's first coefficient, or the zero pad after row 15)
+  for (int e = threadIdx.x; e < 16 * 128; e += blockDim.x) {
+    const int j = e >> 7, m = e & 127;
+    L.d2[e] = kDct[j * 127 + m];
+  }
+  __syncthreads();
+  const uint32_t img = blockIdx.x * kWavesPerBlock + wv;
+  if (img >= n) return;
+  hash_one(planes, plane_stride, img, lane, wv, mode, out, L);
 }
 
 // Dihedrals of given PDQFs (the visit entry point): pdqf [n][256], thr_in [n].
@@ -418,9 +431,10 @@ hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, 
   return hipGetLastError();
 }
 
-template __global__ void pdq_hash_kernel<float>(const float* __restrict__, uint64_t, uint32_t, uint32_t, HashOutputs);
-template __global__ void pdq_hash_kernel<uint8_t>(const uint8_t* __restrict__, uint64_t, uint32_t, uint32_t,
-                                                  HashOutputs);
+template __global__ void pdq_hash_kernel<float>(const float* __restrict__, uint64_t, uint32_t, const uint32_t*,
+                                                uint32_t, HashOutputs);
+template __global__ void pdq_hash_kernel<uint8_t>(const uint8_t* __restrict__, uint64_t, uint32_t, const uint32_t*,
+                                                  uint32_t, HashOutputs);
 
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
                        const HashOutputs& out, hipStream_t stream) {
@@ -429,10 +443,24 @@ hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride
   const dim3 block(64 * kWavesPerBlock);
   if (input_kind == kInputF32)
     hipLaunchKernelGGL(pdq_hash_kernel<float>, grid, block, 0, stream, static_cast<const float*>(planes),
-                       plane_stride, n_images, mode, out);
+                       plane_stride, n_images, nullptr, mode, out);
   else
     hipLaunchKernelGGL(pdq_hash_kernel<uint8_t>, grid, block, 0, stream, static_cast<const uint8_t*>(planes),
-                       plane_stride, n_images, mode, out);
+                       plane_stride, n_images, nullptr, mode, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                                    const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream) {
+  if (n_max == 0) return hipSuccess;
+  const dim3 g((n_max + kWavesPerBlock - 1) / kWavesPerBlock);
+  const dim3 block(64 * kWavesPerBlock);
+  if (input_kind == kInputF32)
+    hipLaunchKernelGGL(pdq_hash_kernel<float>, g, block, 0, stream, static_cast<const float*>(planes), plane_stride,
+                       n_max, d_n, mode, out);
+  else
+    hipLaunchKernelGGL(pdq_hash_kernel<uint8_t>, g, block, 0, stream, static_cast<const uint8_t*>(planes),
+                       plane_stride, n_max, d_n, mode, out);
   return hipGetLastError();
 }
 

@@ -155,6 +155,95 @@ __global__ void __launch_bounds__(kBlock) pdq_match_plane_kernel(const uint4* __
   }
 }
 
+// One pass of kRows rows per lane (row base + u * kBlock) of one shard, over
+// the prefix plane or the raw rows.
+template <int kRows, int kBlock, bool kPlane>
+__device__ __forceinline__ void valu_rows(const uint4* __restrict__ db, const uint4* __restrict__ plane,
+                                          uint64_t n_rows, uint64_t base, const uint32_t* __restrict__ queries,
+                                          uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
+                                          uint64_t capacity, unsigned long long* __restrict__ counter,
+                                          uint64_t index_base) {
+  uint4 pre[kRows];
+  bool valid[kRows];
+#pragma unroll
+  for (int u = 0; u < kRows; u++) {
+    const uint64_t r = base + (uint64_t)u * kBlock;
+    valid[u] = r < n_rows;
+    pre[u] = valid[u] ? (kPlane ? plane[r] : db[2 * r]) : make_uint4(0, 0, 0, 0);
+  }
+  for (uint32_t qi = 0; qi < n_queries; qi++) {
+    const uint32_t* q = queries + 8 * qi;
+#pragma unroll
+    for (int u = 0; u < kRows; u++) {
+      const uint32_t d0 = dist128(pre[u], q);
+      if (__builtin_expect(d0 <= threshold && valid[u], 0)) {
+        const uint64_t r = base + (uint64_t)u * kBlock;
+        const uint32_t d = d0 + dist128(db[2 * r + 1], q + 4);
+        if (d <= threshold) {
+          const unsigned long long slot = atomicAdd(counter, 1ull);
+          if (slot < capacity) {
+            MatchHit h;
+            h.query = qi;
+            h.distance = d;
+            h.index = index_base + r;
+            hits[slot] = h;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Persistent form over a set of database shards (work item = kBlock * kRows
+// rows of one shard), the query count read from device memory; batches of
+// max_nq or more queries are left to the matrix-core kernel.
+template <int kRows, int kBlock, bool kPlane>
+__global__ void __launch_bounds__(kBlock) pdq_match_shards_kernel(const ShardView* __restrict__ shards,
+                                                                  uint32_t n_shards, uint64_t total_rblocks,
+                                                                  const uint32_t* __restrict__ queries,
+                                                                  const uint32_t* __restrict__ d_nq, uint32_t nq_cap,
+                                                                  uint32_t max_nq, uint32_t threshold,
+                                                                  MatchHit* __restrict__ hits, uint64_t capacity,
+                                                                  unsigned long long* __restrict__ counter) {
+  const uint32_t nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+  if (nq == 0 || nq >= max_nq) return;
+  for (uint64_t item = blockIdx.x; item < total_rblocks; item += gridDim.x) {
+    uint32_t sh = 0;
+    while (sh + 1 < n_shards && shards[sh + 1].rblock_begin <= item) sh++;
+    const ShardView& v = shards[sh];
+    const uint64_t base = (item - v.rblock_begin) * (uint64_t)(kBlock * kRows) + threadIdx.x;
+    valu_rows<kRows, kBlock, kPlane>(v.rows, v.plane, v.n_rows, base, queries, nq, threshold, hits, capacity,
+                                     counter, v.index_base);
+  }
+}
+
+uint64_t valu_rblocks(const MatchTuning& t, uint64_t n_rows) {
+  const uint64_t per = (uint64_t)t.block * t.rows_per_lane;
+  return (n_rows + per - 1) / per;
+}
+
+hipError_t launch_match_valu_shards(const MatchTuning& t, const ShardView* shards, uint32_t n_shards,
+                                    uint64_t total_rblocks, bool plane, const void* queries, const uint32_t* d_nq,
+                                    uint32_t nq_cap, uint32_t max_nq, uint32_t threshold, MatchHit* hits,
+                                    uint64_t capacity, unsigned long long* counter, uint32_t grid, hipStream_t stream) {
+  if (grid == 0) return hipErrorInvalidValue;
+#define YPDQ_SHARDS(R, B)                                                                                           \
+  if (t.rows_per_lane == R && t.block == B) {                                                                       \
+    if (plane)                                                                                                      \
+      hipLaunchKernelGGL((pdq_match_shards_kernel<R, B, true>), dim3(grid), dim3(B), 0, stream, shards, n_shards,   \
+                         total_rblocks, static_cast<const uint32_t*>(queries), d_nq, nq_cap, max_nq, threshold,      \
+                         hits, capacity, counter);                                                                  \
+    else                                                                                                            \
+      hipLaunchKernelGGL((pdq_match_shards_kernel<R, B, false>), dim3(grid), dim3(B), 0, stream, shards, n_shards,  \
+                         total_rblocks, static_cast<const uint32_t*>(queries), d_nq, nq_cap, max_nq, threshold,      \
+                         hits, capacity, counter);                                                                  \
+    return hipGetLastError();                                                                                       \
+  }
+  YPDQ_SHARDS(1, 256) YPDQ_SHARDS(2, 256) YPDQ_SHARDS(4, 256) YPDQ_SHARDS(2, 512) YPDQ_SHARDS(4, 512)
+#undef YPDQ_SHARDS
+  return hipErrorInvalidValue;
+}
+
 static int device_cus() {
   static int cus = 0;
   if (!cus) {

@@ -206,29 +206,24 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
   }
 }
 
-// Grid: one dimension of row_chunks x query_groups blocks (row chunks of
-// kMfmaWaves * kMfmaTilesPerWave * 16 rows, query groups of QT * 16 queries).
-// Consecutive workgroups are dealt round-robin to the 8 XCDs, each with its
-// own L2; block b is mapped so that all query groups of a row chunk run on
-// the same XCD (row chunk = xcd + 8 * (slot / groups)) and read its rows from
-// that XCD's L2 after the first one brought them in from HBM.
+// One work item: row chunk `chunk` of a database (kMfmaWaves waves x
+// kMfmaTilesPerWave tiles x 16 rows; each wave streams its own rows) against
+// query group `group` (QT x 16 queries).
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_kernel(
-    const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
-    const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
-    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
-  __shared__ uint32_t lut[256];
-  lut[threadIdx.x] = spread_byte(threadIdx.x);  // blockDim.x == 256
-  __syncthreads();
-  const uint32_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const uint64_t chunk = xcd + 8ull * (slot / groups);
-  const uint32_t group = slot % groups;
-
+__device__ __forceinline__ void mfma_item(const uint32_t* __restrict__ lut, const uint32_t* __restrict__ db,
+                                          const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t chunk,
+                                          uint32_t group, const uint32_t* __restrict__ queries, uint32_t n_queries,
+                                          uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
+                                          unsigned long long* __restrict__ counter, uint64_t index_base) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rn = lane & 15;  // operand row (query m / db row n) of this lane
   const int kq = lane >> 4;  // K quarter = word index of the prefix
   const float bound = (float)(128 - 2 * (int)threshold);
+
+  const uint64_t wave_rows = (uint64_t)kMfmaTilesPerWave * 16;
+  const uint64_t row0 = (chunk * kMfmaWaves + wave) * wave_rows;
+  if (row0 >= n_rows) return;
 
   // query operands: tile t, query qbase + 16t + rn, word kq
   const uint32_t qbase = group * (uint32_t)(QT * 16);
@@ -248,9 +243,6 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
       qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;  // zero rows never reach the bound
   }
 
-  const uint64_t wave_rows = (uint64_t)kMfmaTilesPerWave * 16;
-  const uint64_t row0 = (chunk * kMfmaWaves + wave) * wave_rows;
-  if (row0 >= n_rows) return;
   const RowSource<kPlane> src(db, plane, row0, lane);
   // Only the wave that reaches the end of the database needs bounds checks.
   if (row0 + wave_rows <= n_rows)
@@ -259,6 +251,67 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
   else
     stream_tiles<QT, false, kPlane>(qa, lut, src, db, row0, n_rows, qbase, rn, kq, bound, queries, n_queries,
                                     threshold, hits, capacity, counter, index_base);
+}
+
+__device__ __forceinline__ void init_lut(uint32_t* lut) {
+  lut[threadIdx.x] = spread_byte(threadIdx.x);  // blockDim.x == 256
+  __syncthreads();
+}
+
+// Work item -> (row chunk, query group).  Consecutive workgroups are dealt
+// round-robin to the 8 XCDs, each with its own L2; items are numbered so that
+// all query groups of a row chunk run on the same XCD (row chunk = xcd +
+// 8 * (slot / groups)) and read its rows from that XCD's L2 after the first
+// one brought them in from HBM.
+__device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint64_t& chunk, uint32_t& group) {
+  const uint64_t xcd = item & 7, slot = item >> 3;
+  chunk = xcd + 8 * (slot / groups);
+  group = (uint32_t)(slot % groups);
+}
+
+// One launch, one block per work item (row chunks x query groups).
+template <int QT, bool kPlane>
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_kernel(
+    const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
+    const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
+    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
+  __shared__ uint32_t lut[256];
+  init_lut(lut);
+  uint64_t chunk;
+  uint32_t group;
+  item_coords(blockIdx.x, groups, chunk, group);
+  mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity, counter,
+                        index_base);
+}
+
+// Persistent form over a set of database shards: a fixed grid (a multiple of
+// 8 blocks) walks the work items of every shard, blockIdx.x + k * gridDim.x,
+// which keeps each item on the XCD its number selects.  The query count is
+// read from device memory (`d_nq`, capped at nq_cap), so one captured launch
+// serves batches of any size; batches of fewer than min_nq queries are left
+// to the vector-ALU kernel and this one returns at once.
+template <int QT, bool kPlane>
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_shards_kernel(
+    const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
+    const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t min_nq,
+    uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter) {
+  const uint32_t nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+  if (nq == 0 || nq < min_nq) return;
+  __shared__ uint32_t lut[256];
+  init_lut(lut);
+  const uint32_t groups = (nq + QT * 16 - 1) / (QT * 16);
+  const uint64_t items = total_chunks * groups;
+  for (uint64_t item = blockIdx.x; item < items; item += gridDim.x) {
+    uint64_t chunk;
+    uint32_t group;
+    item_coords(item, groups, chunk, group);
+    uint32_t sh = 0;
+    while (sh + 1 < n_shards && shards[sh + 1].chunk_begin <= chunk) sh++;
+    const ShardView& v = shards[sh];
+    mfma_item<QT, kPlane>(lut, reinterpret_cast<const uint32_t*>(v.rows), reinterpret_cast<const uint32_t*>(v.plane),
+                          v.n_rows, chunk - v.chunk_begin, group, queries, nq, threshold, hits, capacity, counter,
+                          v.index_base);
+  }
 }
 
 template <int QT>
@@ -303,6 +356,45 @@ hipError_t launch_build_prefix_plane(const void* db, uint64_t n_rows, void* plan
   hipLaunchKernelGGL(build_prefix_plane_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
                      static_cast<const uint4*>(db), n_rows, static_cast<uint4*>(plane), n_padded);
   return hipGetLastError();
+}
+
+uint64_t mfma_chunks(uint64_t n_rows) {
+  const uint64_t rows_per_block = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
+  return ((n_rows + rows_per_block - 1) / rows_per_block + 7) / 8 * 8;
+}
+
+template <int QT>
+static hipError_t launch_shards_qt(const ShardView* shards, uint32_t n_shards, uint64_t total_chunks, bool plane,
+                                   const void* queries, const uint32_t* d_nq, uint32_t nq_cap, uint32_t min_nq,
+                                   uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
+                                   uint32_t grid, hipStream_t stream) {
+  if (plane)
+    hipLaunchKernelGGL((pdq_match_mfma_shards_kernel<QT, true>), dim3(grid), dim3(64 * kMfmaWaves), 0, stream, shards,
+                       n_shards, total_chunks, static_cast<const uint32_t*>(queries), d_nq, nq_cap, min_nq, threshold,
+                       hits, capacity, counter);
+  else
+    hipLaunchKernelGGL((pdq_match_mfma_shards_kernel<QT, false>), dim3(grid), dim3(64 * kMfmaWaves), 0, stream,
+                       shards, n_shards, total_chunks, static_cast<const uint32_t*>(queries), d_nq, nq_cap, min_nq,
+                       threshold, hits, capacity, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_match_mfma_shards(int qt, const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
+                                    bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                    uint32_t min_nq, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                    unsigned long long* counter, uint32_t grid, hipStream_t stream) {
+  if (threshold >= 64 || grid == 0 || grid % 8) return hipErrorInvalidValue;
+  switch (qt) {
+    case 2: return launch_shards_qt<2>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, min_nq, threshold,
+                                       hits, capacity, counter, grid, stream);
+    case 4: return launch_shards_qt<4>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, min_nq, threshold,
+                                       hits, capacity, counter, grid, stream);
+    case 8: return launch_shards_qt<8>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, min_nq, threshold,
+                                       hits, capacity, counter, grid, stream);
+    case 16: return launch_shards_qt<16>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, min_nq,
+                                         threshold, hits, capacity, counter, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold) { return threshold < 64 && n_queries >= 32; }

@@ -40,6 +40,17 @@ class Hit(C.Structure):
 
 DihedralCallback = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_uint32, C.c_float, C.c_float)
 
+
+class BatchBuffers(C.Structure):
+    _fields_ = [("d_input", C.c_void_p), ("input_stride_bytes", C.c_uint64), ("d_hashes", C.c_void_p),
+                ("d_quality", C.c_void_p), ("d_threshold", C.c_void_p), ("d_hits", C.c_void_p),
+                ("hit_capacity", C.c_uint64), ("d_count", C.c_void_p), ("stream", C.c_void_p),
+                ("max_images", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+class TuneRecord(C.Structure):
+    _fields_ = [("n_queries", C.c_uint32), ("path", C.c_uint32), ("query_tiles", C.c_uint32), ("ms", C.c_float)]
+
 _lib = None
 
 
@@ -74,6 +85,27 @@ def lib():
         L.yume_pdq_amd_prefix_plane_bytes.argtypes = [u64]
         L.yume_pdq_amd_build_prefix_plane.restype = i
         L.yume_pdq_amd_build_prefix_plane.argtypes = [vp, u64, vp, vp]
+        pp = C.POINTER(C.c_void_p)
+        for name, res, args in (
+                ("yume_pdq_amd_shards_create", i, [pp]),
+                ("yume_pdq_amd_shards_destroy", None, [vp]),
+                ("yume_pdq_amd_shards_add", i, [vp, vp, u64, u64, i, C.POINTER(C.c_uint32)]),
+                ("yume_pdq_amd_shards_load_raw", i, [vp, C.c_char_p, u64, u64, C.POINTER(C.c_uint32)]),
+                ("yume_pdq_amd_shards_replace", i, [vp, u32, vp, vp, u32, vp, vp]),
+                ("yume_pdq_amd_shards_info", i, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                                 C.POINTER(C.c_uint64)]),
+                ("yume_pdq_amd_shards_match", i, [vp, vp, u32, u32, vp, u64, vp, u32, vp]),
+                ("yume_pdq_amd_batcher_create", i, [vp, u32, i, u32, u32, u64, pp]),
+                ("yume_pdq_amd_batcher_destroy", None, [vp]),
+                ("yume_pdq_amd_batcher_buffers", i, [vp, C.POINTER(BatchBuffers)]),
+                ("yume_pdq_amd_batcher_run", i, [vp, u32, i]),
+                ("yume_pdq_amd_batcher_autotune", i, [vp, i]),
+                ("yume_pdq_amd_batcher_tune_records", i, [vp, C.POINTER(TuneRecord), u32,
+                                                          C.POINTER(C.c_uint32)]),
+                ("yume_pdq_amd_batcher_choice", i, [vp, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)])):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         L.yume_pdq_amd_match_path.restype = i
         L.yume_pdq_amd_match_path.argtypes = [u32, u32, u32]
         for name, args in (("compress", [vp, i, u64, u32, vp, vp]),
@@ -404,3 +436,191 @@ def stage_luma(pixels, coeff=RGB8_TO_LUMA8_TABLE_ITU, stream=None):
     _check(lib().yume_pdq_amd_stage_luma(_vp(pixels), ch, out.numel(), float(coeff[0]), float(coeff[1]),
                                          float(coeff[2]), _vp(out), _stream_ptr(stream)), "stage_luma")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batching layer: shard sets and batch contexts (include/yume_pdq_amd.h)
+# ---------------------------------------------------------------------------
+class _DeviceArray:
+    """A device allocation owned by the library, seen by torch through the
+    CUDA array interface (no copy)."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr,
+                                         "data": (int(ptr), False), "version": 3, "strides": None}
+
+
+def _alias(ptr, shape, dtype):
+    torch = _torch()
+    typestr = {torch.uint8: "|u1", torch.float32: "<f4", torch.int64: "<i8"}[dtype]
+    return torch.as_tensor(_DeviceArray(ptr, shape, typestr), device="cuda")
+
+
+def _hits_to_numpy(hits_u8, total, capacity):
+    k = min(int(total), capacity)
+    arr = hits_u8[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
+    return np.sort(arr, order=("index", "query"))
+
+
+class Shards:
+    """Database shards resident in HBM (rows + prefix plane each)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().yume_pdq_amd_shards_create(C.byref(h)), "shards_create")
+        self._h = h
+        self._next_base = 0
+        self._keep = []  # device tensors referenced by shards
+
+    def close(self):
+        if self._h:
+            lib().yume_pdq_amd_shards_destroy(self._h)
+            self._h = None
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, rows, index_base=None):
+        """rows: host numpy (n, 32) u8 (copied into HBM) or CUDA u8 tensor
+        (referenced; kept alive by this object).  Returns the shard id."""
+        torch = _torch()
+        base = self._next_base if index_base is None else int(index_base)
+        sid = C.c_uint32(0)
+        if isinstance(rows, np.ndarray):
+            r = np.ascontiguousarray(rows, np.uint8).reshape(-1, 32)
+            _check(lib().yume_pdq_amd_shards_add(self._h, r.ctypes.data_as(C.c_void_p), r.shape[0], base, 0,
+                                                 C.byref(sid)), "shards_add")
+            n = r.shape[0]
+        else:
+            r = rows.reshape(-1, 32).contiguous()
+            if r.dtype != torch.uint8 or not r.is_cuda:
+                raise ValueError("device rows must be a CUDA uint8 tensor")
+            torch.cuda.synchronize()
+            _check(lib().yume_pdq_amd_shards_add(self._h, C.c_void_p(r.data_ptr()), r.shape[0], base, 1,
+                                                 C.byref(sid)), "shards_add")
+            self._keep.append(r)
+            n = r.shape[0]
+        self._next_base = max(self._next_base, base + n)
+        return sid.value
+
+    def load_raw(self, path, rows_per_shard, index_base=0):
+        n = C.c_uint32(0)
+        _check(lib().yume_pdq_amd_shards_load_raw(self._h, str(path).encode(), int(rows_per_shard),
+                                                  int(index_base), C.byref(n)), "shards_load_raw")
+        return n.value
+
+    def replace(self, shard, local_rows, new_rows):
+        """Rows local_rows (CUDA int64) of `shard` become new_rows (CUDA u8 (n, 32));
+        returns the number of out-of-range indices skipped."""
+        torch = _torch()
+        idx = local_rows.reshape(-1).to(torch.int64).contiguous()
+        new = new_rows.reshape(-1, 32).contiguous()
+        bad = torch.zeros(1, dtype=torch.int64, device=new.device)
+        _check(lib().yume_pdq_amd_shards_replace(self._h, int(shard), C.c_void_p(idx.data_ptr()),
+                                                 C.c_void_p(new.data_ptr()), idx.numel(),
+                                                 C.c_void_p(bad.data_ptr()), _stream_ptr(None)),
+               "shards_replace")
+        return int(bad.item())
+
+    def info(self):
+        n, rows, b = C.c_uint32(0), C.c_uint64(0), C.c_uint64(0)
+        _check(lib().yume_pdq_amd_shards_info(self._h, C.byref(n), C.byref(rows), C.byref(b)), "shards_info")
+        return {"shards": n.value, "rows": rows.value, "device_bytes": b.value}
+
+    def match(self, queries, threshold, capacity=1 << 16, path=None):
+        """Exhaustive match against every shard (one persistent launch):
+        (total, hits sorted by (index, query)) with global indices."""
+        torch = _torch()
+        q = queries.reshape(-1, 32).contiguous()
+        hits = torch.empty((max(capacity, 1), 16), dtype=torch.uint8, device=q.device)
+        count = torch.zeros(1, dtype=torch.int64, device=q.device)
+        flags = {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path]
+        _check(lib().yume_pdq_amd_shards_match(self._h, C.c_void_p(q.data_ptr()), q.shape[0], int(threshold),
+                                               C.c_void_p(hits.data_ptr()), hits.shape[0],
+                                               C.c_void_p(count.data_ptr()), flags, _stream_ptr(None)),
+               "shards_match")
+        total = int(count.item())
+        return total, _hits_to_numpy(hits, total, hits.shape[0])
+
+
+class Batcher:
+    """Batch context: hash up to max_images planes and match their 8n hashes
+    against every shard of `shards`, replayed as one HIP graph per batch."""
+
+    def __init__(self, shards, max_images, input_kind=INPUT_U8, dihedral_mode=DIHEDRAL_EXACT, threshold=31,
+                 hit_capacity=1 << 20):
+        h = C.c_void_p()
+        _check(lib().yume_pdq_amd_batcher_create(shards._h, int(max_images), int(input_kind), int(dihedral_mode),
+                                                 int(threshold), int(hit_capacity), C.byref(h)), "batcher_create")
+        self._h = h
+        self._shards = shards  # keep the set alive
+        b = BatchBuffers()
+        _check(lib().yume_pdq_amd_batcher_buffers(h, C.byref(b)), "batcher_buffers")
+        self.buffers = b
+        torch = _torch()
+        self.max_images = b.max_images
+        elem = torch.float32 if input_kind == INPUT_F32 else torch.uint8
+        self.input = _alias(b.d_input, (b.max_images, 512, 512), elem)
+        self.hashes = _alias(b.d_hashes, (b.max_images, 8, 32), torch.uint8)
+        self.quality = _alias(b.d_quality, (b.max_images, 8), torch.float32)
+        self.threshold = _alias(b.d_threshold, (b.max_images, 8), torch.float32)
+        self.hits = _alias(b.d_hits, (b.hit_capacity, 16), torch.uint8)
+        self.count = _alias(b.d_count, (1,), torch.int64)
+        self.stream = torch.cuda.ExternalStream(b.stream)
+
+    def close(self):
+        if self._h:
+            self.stream.synchronize()
+            lib().yume_pdq_amd_batcher_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, planes, use_graph=True, sync=True):
+        """Copy n planes (CUDA tensor (n, 512, 512)) in and run one batch.
+        With sync, returns (hashes (n,8,32), quality, threshold, total, hits)."""
+        torch = _torch()
+        n = planes.shape[0]
+        if n > self.max_images:
+            raise ValueError("batch larger than max_images")
+        torch.cuda.current_stream().synchronize()  # planes written on torch's stream
+        with torch.cuda.stream(self.stream):
+            self.input[:n].copy_(planes)
+        _check(lib().yume_pdq_amd_batcher_run(self._h, n, 1 if use_graph else 0), "batcher_run")
+        if not sync:
+            return None
+        return self.results(n)
+
+    def results(self, n):
+        torch = _torch()
+        with torch.cuda.stream(self.stream):
+            h = self.hashes[:n].clone()
+            q = self.quality[:n].clone()
+            t = self.threshold[:n].clone()
+            c = self.count.clone()
+        self.stream.synchronize()
+        total = int(c.item())
+        return h, q, t, total, _hits_to_numpy(self.hits, total, self.hits.shape[0])
+
+    def autotune(self, iterations=5):
+        _check(lib().yume_pdq_amd_batcher_autotune(self._h, int(iterations)), "batcher_autotune")
+        n = C.c_uint32(0)
+        _check(lib().yume_pdq_amd_batcher_tune_records(self._h, None, 0, C.byref(n)), "tune_records")
+        recs = (TuneRecord * max(n.value, 1))()
+        _check(lib().yume_pdq_amd_batcher_tune_records(self._h, recs, n.value, C.byref(n)), "tune_records")
+        return [{"n_queries": r.n_queries, "path": "mfma" if r.path == MATCH_MFMA else "valu",
+                 "query_tiles": r.query_tiles, "ms": r.ms} for r in recs[:n.value]]
+
+    def choice(self, n_queries):
+        path, qt = C.c_uint32(0), C.c_uint32(0)
+        _check(lib().yume_pdq_amd_batcher_choice(self._h, int(n_queries), C.byref(path), C.byref(qt)),
+               "batcher_choice")
+        return ("mfma" if path.value == MATCH_MFMA else "valu"), qt.value
