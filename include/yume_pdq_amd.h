@@ -26,7 +26,8 @@ enum {
   YUME_PDQ_AMD_EHIP = -2,      /* a HIP runtime call failed */
   YUME_PDQ_AMD_ENOMEM = -3,    /* device allocation failed */
   YUME_PDQ_AMD_ENODEV = -4,    /* no usable GPU */
-  YUME_PDQ_AMD_EOVERFLOW = -5  /* more hits than the buffer holds (count still exact) */
+  YUME_PDQ_AMD_EOVERFLOW = -5, /* more hits than the buffer holds (count still exact) */
+  YUME_PDQ_AMD_ETIMEOUT = -6   /* a wait ran out of time */
 };
 
 enum { YUME_PDQ_AMD_INPUT_F32 = 0, YUME_PDQ_AMD_INPUT_U8 = 1 };
@@ -272,6 +273,56 @@ int yume_pdq_amd_batcher_tune_records(const YumePdqAmdBatcher *b, YumePdqAmdTune
                                       uint32_t cap, uint32_t *n);
 int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher *b, uint32_t n_queries, uint32_t *path,
                                 uint32_t *query_tiles);
+
+/*
+ * Serving loop (continuous batching).  No reference counterpart.  Requests of
+ * 1..max_batch_images u8 planes are copied into pinned staging by submit()
+ * and queued; a worker thread admits queued requests in arrival order into
+ * the next free batch context (as many contexts as hbm_budget_bytes holds,
+ * at most max_contexts, so batches overlap on the device), replays its
+ * hash -> match graph, and hands each request its own hashes, qualities,
+ * thresholds and hits.  A request's results do not depend on which other
+ * requests shared its batch.  Hit queries are 8 * (image in the request) +
+ * dihedral; indices are global.
+ */
+typedef struct YumePdqAmdServer YumePdqAmdServer;
+typedef struct YumePdqAmdServerConfig {
+  uint32_t max_batch_images;
+  uint64_t hbm_budget_bytes;
+  uint32_t max_contexts;
+  uint32_t threshold;
+  uint32_t dihedral_mode;
+  uint64_t hit_capacity;      /* per batch (a larger batch result is re-matched) */
+  int use_graph;
+} YumePdqAmdServerConfig;
+typedef struct YumePdqAmdResult {
+  int status;                 /* YUME_PDQ_AMD_OK or an error of the batch */
+  uint32_t n_images;
+  const uint8_t *hashes;      /* n_images x 8 x 32 */
+  const float *quality;       /* n_images x 8 */
+  const float *threshold;     /* n_images x 8 */
+  const YumePdqAmdHit *hits;  /* sorted by (index, query) */
+  uint64_t n_hits;
+  double latency_ms;          /* submit -> results ready */
+  uint64_t batch;             /* sequence number of the serving batch */
+  uint32_t batch_images;      /* images in that batch */
+} YumePdqAmdResult;
+typedef struct YumePdqAmdServerStats {
+  uint32_t contexts;
+  uint64_t device_bytes;
+  uint64_t batches;
+  uint64_t images;
+} YumePdqAmdServerStats;
+int yume_pdq_amd_server_create(YumePdqAmdShards *shards, const YumePdqAmdServerConfig *cfg,
+                               YumePdqAmdServer **out);
+void yume_pdq_amd_server_destroy(YumePdqAmdServer *server);
+int yume_pdq_amd_server_submit(YumePdqAmdServer *server, const uint8_t *planes, uint32_t n_images,
+                               uint64_t *request_id);
+/* Result pointers stay valid until yume_pdq_amd_server_release(id). */
+int yume_pdq_amd_server_wait(YumePdqAmdServer *server, uint64_t request_id, int timeout_ms,
+                             YumePdqAmdResult *out);
+int yume_pdq_amd_server_release(YumePdqAmdServer *server, uint64_t request_id);
+int yume_pdq_amd_server_stats(const YumePdqAmdServer *server, YumePdqAmdServerStats *out);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -199,6 +199,7 @@ const char* yume_pdq_amd_strerror(int code) {
     case YUME_PDQ_AMD_ENOMEM: return "device out of memory";
     case YUME_PDQ_AMD_ENODEV: return "no usable GPU";
     case YUME_PDQ_AMD_EOVERFLOW: return "hit buffer overflow";
+    case YUME_PDQ_AMD_ETIMEOUT: return "timed out";
     default: return "unknown error";
   }
 }

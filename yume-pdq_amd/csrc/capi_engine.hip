@@ -4,15 +4,24 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "../../include/yume_pdq_amd.h"
 #include "engine.h"
+#include "server.h"
 
 struct YumePdqAmdShards {
   std::shared_ptr<ypdq::ShardSet> set;
+};
+
+struct YumePdqAmdServer {
+  ypdq::Server server;
+  std::mutex mu;
+  std::map<uint64_t, ypdq::RequestResult> held;  // results handed out, until released
 };
 
 struct YumePdqAmdBatcher {
@@ -168,6 +177,70 @@ int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher* b, uint32_t n_queries, 
   const ypdq::MatcherChoice c = b->ctx->choice_for(n_queries);
   *path = c.mfma ? YUME_PDQ_AMD_MATCH_MFMA : YUME_PDQ_AMD_MATCH_VALU;
   *query_tiles = c.mfma ? (uint32_t)c.qt : 0;
+  return YUME_PDQ_AMD_OK;
+}
+
+int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig* cfg, YumePdqAmdServer** out) {
+  if (!s || !cfg || !out) return YUME_PDQ_AMD_EINVAL;
+  YumePdqAmdServer* v = new (std::nothrow) YumePdqAmdServer();
+  if (!v) return YUME_PDQ_AMD_ENOMEM;
+  ypdq::ServerConfig c;
+  c.max_batch_images = cfg->max_batch_images;
+  c.hbm_budget_bytes = cfg->hbm_budget_bytes;
+  c.max_contexts = cfg->max_contexts;
+  c.threshold = cfg->threshold;
+  c.dihedral_mode = cfg->dihedral_mode;
+  c.hit_capacity = cfg->hit_capacity;
+  c.use_graph = cfg->use_graph != 0;
+  const hipError_t e = v->server.start(s->set, c);
+  if (e != hipSuccess) {
+    delete v;
+    return status(e);
+  }
+  *out = v;
+  return YUME_PDQ_AMD_OK;
+}
+
+void yume_pdq_amd_server_destroy(YumePdqAmdServer* v) { delete v; }
+
+int yume_pdq_amd_server_submit(YumePdqAmdServer* v, const uint8_t* planes, uint32_t n_images, uint64_t* id) {
+  if (!v || !planes || !id) return YUME_PDQ_AMD_EINVAL;
+  *id = v->server.submit(planes, n_images);
+  return *id ? YUME_PDQ_AMD_OK : YUME_PDQ_AMD_EINVAL;
+}
+
+int yume_pdq_amd_server_wait(YumePdqAmdServer* v, uint64_t id, int timeout_ms, YumePdqAmdResult* out) {
+  if (!v || !out) return YUME_PDQ_AMD_EINVAL;
+  ypdq::RequestResult r;
+  if (!v->server.wait(id, timeout_ms, &r)) return YUME_PDQ_AMD_ETIMEOUT;
+  std::lock_guard<std::mutex> g(v->mu);
+  ypdq::RequestResult& h = v->held[id];
+  h = std::move(r);
+  out->status = h.status;
+  out->n_images = h.n_images;
+  out->hashes = h.hashes.data();
+  out->quality = h.quality.data();
+  out->threshold = h.threshold.data();
+  out->hits = reinterpret_cast<const YumePdqAmdHit*>(h.hits.data());
+  out->n_hits = h.hits.size();
+  out->latency_ms = h.latency_ms;
+  out->batch = h.batch;
+  out->batch_images = h.batch_images;
+  return YUME_PDQ_AMD_OK;
+}
+
+int yume_pdq_amd_server_release(YumePdqAmdServer* v, uint64_t id) {
+  if (!v) return YUME_PDQ_AMD_EINVAL;
+  std::lock_guard<std::mutex> g(v->mu);
+  return v->held.erase(id) ? YUME_PDQ_AMD_OK : YUME_PDQ_AMD_EINVAL;
+}
+
+int yume_pdq_amd_server_stats(const YumePdqAmdServer* v, YumePdqAmdServerStats* out) {
+  if (!v || !out) return YUME_PDQ_AMD_EINVAL;
+  out->contexts = v->server.contexts();
+  out->device_bytes = v->server.device_bytes();
+  out->batches = v->server.batches();
+  out->images = v->server.images_served();
   return YUME_PDQ_AMD_OK;
 }
 

@@ -92,8 +92,9 @@ class BatchContext {
                   uint32_t threshold, uint64_t hit_capacity);
   // Hash input[0 .. n_images) and match the 8n hashes against every shard,
   // enqueued on stream() (a graph replay unless use_graph is false).  Waits
-  // for this context's previous batch first.
-  hipError_t run(uint32_t n_images, bool use_graph);
+  // for this context's previous batch first unless the caller knows it is
+  // done (wait_previous = false).
+  hipError_t run(uint32_t n_images, bool use_graph, bool wait_previous = true);
   hipError_t autotune(int iterations, std::vector<TuneRecord>* records);
   MatcherChoice choice_for(uint32_t n_queries) const;
 

@@ -311,13 +311,13 @@ hipError_t BatchContext::graph_for(const MatcherChoice& c, hipGraphExec_t* exec)
   return hipSuccess;
 }
 
-hipError_t BatchContext::run(uint32_t n_images, bool use_graph) {
+hipError_t BatchContext::run(uint32_t n_images, bool use_graph, bool wait_previous) {
   if (n_images > max_images_) return hipErrorInvalidValue;
   const MatcherChoice c = choice_for(n_images * 8);
-  // the batch size goes to device memory through pinned staging; the previous
-  // batch on this stream has consumed its copy by the time this one is read,
-  // but its upload may still be in flight, so wait for it before rewriting
-  hipError_t e = hipStreamSynchronize(stream_);
+  // the batch size goes to device memory through pinned staging, which the
+  // previous batch's upload may still be reading: wait for that batch before
+  // rewriting it
+  hipError_t e = wait_previous ? hipStreamSynchronize(stream_) : hipSuccess;
   if (e != hipSuccess) return e;
   h_sizes_[0] = n_images;
   h_sizes_[1] = n_images * 8;

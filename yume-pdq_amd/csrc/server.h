@@ -1,0 +1,112 @@
+// Serving loop: continuous batching of hash + match requests over a shard set.
+// See server.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+namespace ypdq {
+
+struct ServerConfig {
+  uint32_t max_batch_images = 64;
+  uint64_t hbm_budget_bytes = 0;  // device memory the batch contexts may hold
+  uint32_t max_contexts = 4;      // batches in flight at most
+  uint32_t threshold = 31;
+  uint32_t dihedral_mode = kDihedralExact;
+  uint64_t hit_capacity = 1 << 20;  // per batch; a batch with more hits is re-matched into a larger buffer
+  bool use_graph = true;
+};
+
+struct RequestResult {
+  int status = 0;  // 0 ok, negative: error
+  uint32_t n_images = 0;
+  std::vector<uint8_t> hashes;   // n x 8 x 32
+  std::vector<float> quality;    // n x 8
+  std::vector<float> threshold;  // n x 8
+  std::vector<MatchHit> hits;    // query = 8 * image-in-request + dihedral, index global
+  double latency_ms = 0;         // submit -> result ready
+  uint64_t batch = 0;            // sequence number of the batch that served it
+  uint32_t batch_images = 0;     // images in that batch
+};
+
+class Server {
+ public:
+  Server() = default;
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  hipError_t start(std::shared_ptr<ShardSet> shards, const ServerConfig& cfg);
+  void stop();
+  // Copies n u8 planes (n x 512 x 512) into pinned staging; blocks while the
+  // staging ring is full.  Returns the request id (> 0) or 0 on bad input.
+  uint64_t submit(const uint8_t* planes, uint32_t n);
+  // Waits up to timeout_ms (< 0: forever) for a request; true when done.
+  bool wait(uint64_t id, int timeout_ms, RequestResult* out);
+  uint32_t contexts() const { return (uint32_t)ctx_.size(); }
+  uint64_t device_bytes() const;
+  uint64_t batches() const { return batches_.load(); }
+  uint64_t images_served() const { return images_.load(); }
+
+ private:
+  struct Request {
+    uint64_t id;
+    uint32_t n;
+    uint64_t slot;  // first staging slot
+    std::chrono::steady_clock::time_point t_submit;
+  };
+  struct Slot {  // one batch context and the requests it is running
+    std::unique_ptr<BatchContext> ctx;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t batch_no = 0;
+    uint32_t n_images = 0;
+    std::vector<Request> reqs;
+    std::vector<uint32_t> offsets;  // first image of each request in the batch
+    // pinned readback staging
+    uint8_t* h_hashes = nullptr;
+    float* h_quality = nullptr;
+    float* h_threshold = nullptr;
+    unsigned long long* h_count = nullptr;
+    MatchHit* h_hits = nullptr;
+    uint64_t h_hits_cap = 0;
+  };
+  void loop();
+  bool launch_batch(Slot& s);
+  void harvest(Slot& s);
+  void free_slots(uint64_t first, uint32_t n);
+
+  ServerConfig cfg_;
+  std::shared_ptr<ShardSet> shards_;
+  std::vector<std::unique_ptr<Slot>> ctx_;
+  uint64_t slot_bytes_ = 512ull * 512;
+  uint8_t* staging_ = nullptr;  // pinned ring of ring_slots_ images
+  uint64_t ring_slots_ = 0;
+  std::vector<uint8_t> slot_used_;
+  uint64_t ring_head_ = 0;  // next slot to hand out
+
+  std::mutex mu_;
+  std::condition_variable cv_submit_;  // worker: new requests
+  std::condition_variable cv_space_;   // submitters: staging freed
+  std::condition_variable cv_done_;    // waiters: results
+  std::deque<Request> pending_;
+  std::map<uint64_t, RequestResult> results_;
+  uint64_t next_id_ = 1;
+  bool stop_ = false;
+  std::thread worker_;
+  std::atomic<uint64_t> batches_{0}, images_{0};
+};
+
+}  // namespace ypdq

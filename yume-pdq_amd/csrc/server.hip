@@ -1,0 +1,280 @@
+// Serving loop: continuous batching of hash + match requests.
+//
+// Requests (1..max_batch_images u8 planes each) are copied by the submitting
+// thread into a pinned staging ring and queued.  One worker thread owns every
+// GPU call: whenever a batch context is free it admits queued requests in
+// arrival order until the batch is full, uploads their planes, replays the
+// context's hash -> match graph and records an event; whenever a context's
+// event has fired it reads the batch back and hands each request its hashes,
+// qualities, thresholds and hits.  Contexts run on their own streams, so
+// requests that arrive while batches are on the GPU form the next batch and
+// overlap with them.  The number of contexts -- batches in flight -- is what
+// the HBM budget allows (each holds max_batch_images planes, outputs and a
+// hit buffer), capped at max_contexts.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "server.h"
+
+namespace ypdq {
+
+Server::~Server() { stop(); }
+
+uint64_t Server::device_bytes() const {
+  uint64_t t = 0;
+  for (auto& s : ctx_) t += s->ctx->device_bytes();
+  return t;
+}
+
+hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& cfg) {
+  if (worker_.joinable()) return hipErrorInvalidValue;
+  if (cfg.max_batch_images == 0 || cfg.max_batch_images > 4096 || cfg.threshold > 256 || cfg.dihedral_mode > 1 ||
+      cfg.hit_capacity == 0 || cfg.max_contexts == 0)
+    return hipErrorInvalidValue;
+  cfg_ = cfg;
+  shards_ = std::move(shards);
+  hipError_t e;
+  for (uint32_t k = 0; k < std::min<uint32_t>(cfg.max_contexts, 16); k++) {
+    std::unique_ptr<Slot> s(new Slot());
+    s->ctx.reset(new BatchContext());
+    if ((e = s->ctx->init(shards_, cfg.max_batch_images, kInputU8, cfg.dihedral_mode, cfg.threshold,
+                          cfg.hit_capacity)) != hipSuccess)
+      return e;
+    // admission by HBM budget: a context is kept only if it fits
+    if (device_bytes() + s->ctx->device_bytes() > cfg.hbm_budget_bytes) break;
+    if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;
+    const uint32_t m = cfg.max_batch_images;
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_hashes), (size_t)m * 256, hipHostMallocDefault)) ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_quality), (size_t)m * 32, hipHostMallocDefault)) ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_threshold), (size_t)m * 32, hipHostMallocDefault)) ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_count), 8, hipHostMallocDefault)) ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_hits), cfg.hit_capacity * sizeof(MatchHit),
+                           hipHostMallocDefault)))
+      return e;
+    s->h_hits_cap = cfg.hit_capacity;
+    ctx_.push_back(std::move(s));
+  }
+  if (ctx_.empty()) return hipErrorOutOfMemory;  // the budget does not hold one batch
+  ring_slots_ = (uint64_t)(ctx_.size() + 1) * cfg.max_batch_images;
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&staging_), ring_slots_ * slot_bytes_, hipHostMallocDefault)) !=
+      hipSuccess)
+    return e;
+  slot_used_.assign(ring_slots_, 0);
+  stop_ = false;
+  worker_ = std::thread([this] { loop(); });
+  return hipSuccess;
+}
+
+void Server::stop() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_submit_.notify_all();
+  if (worker_.joinable()) worker_.join();
+  for (auto& s : ctx_) {
+    if (s->done) (void)hipEventDestroy(s->done);
+    void* h[] = {s->h_hashes, s->h_quality, s->h_threshold, s->h_count, s->h_hits};
+    for (void* p : h)
+      if (p) (void)hipHostFree(p);
+  }
+  ctx_.clear();
+  if (staging_) (void)hipHostFree(staging_);
+  staging_ = nullptr;
+}
+
+uint64_t Server::submit(const uint8_t* planes, uint32_t n) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!planes || n == 0 || n > cfg_.max_batch_images || !worker_.joinable()) return 0;
+  uint64_t first, id;
+  {
+    std::unique_lock<std::mutex> g(mu_);
+    for (;;) {
+      if (stop_) return 0;
+      first = ring_head_ + n <= ring_slots_ ? ring_head_ : 0;
+      bool free = true;
+      for (uint64_t k = first; k < first + n; k++) free = free && !slot_used_[k];
+      if (free) break;
+      cv_space_.wait(g);
+    }
+    for (uint64_t k = first; k < first + n; k++) slot_used_[k] = 1;
+    ring_head_ = (first + n) % ring_slots_;
+    id = next_id_++;
+  }
+  memcpy(staging_ + first * slot_bytes_, planes, (size_t)n * slot_bytes_);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    pending_.push_back(Request{id, n, first, t0});
+  }
+  cv_submit_.notify_one();
+  return id;
+}
+
+bool Server::wait(uint64_t id, int timeout_ms, RequestResult* out) {
+  std::unique_lock<std::mutex> g(mu_);
+  auto ready = [&] { return results_.count(id) != 0; };
+  if (timeout_ms < 0)
+    cv_done_.wait(g, ready);
+  else if (!cv_done_.wait_for(g, std::chrono::milliseconds(timeout_ms), ready))
+    return false;
+  auto it = results_.find(id);
+  if (out) *out = std::move(it->second);
+  results_.erase(it);
+  return true;
+}
+
+void Server::free_slots(uint64_t first, uint32_t n) {
+  for (uint64_t k = first; k < first + n; k++) slot_used_[k] = 0;
+}
+
+bool Server::launch_batch(Slot& s) {
+  s.reqs.clear();
+  s.offsets.clear();
+  uint32_t total = 0;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    while (!pending_.empty() && total + pending_.front().n <= cfg_.max_batch_images) {
+      s.offsets.push_back(total);
+      total += pending_.front().n;
+      s.reqs.push_back(pending_.front());
+      pending_.pop_front();
+    }
+  }
+  if (total == 0) return false;
+  BatchContext& c = *s.ctx;
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < s.reqs.size() && e == hipSuccess; i++)
+    e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.offsets[i] * slot_bytes_,
+                       staging_ + s.reqs[i].slot * slot_bytes_, (size_t)s.reqs[i].n * slot_bytes_,
+                       hipMemcpyHostToDevice, c.stream());
+  // the context's previous batch was harvested, so there is nothing to wait for
+  if (e == hipSuccess) e = c.run(total, cfg_.use_graph, /*wait_previous=*/false);
+  if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
+  s.n_images = total;
+  s.batch_no = batches_.fetch_add(1) + 1;
+  s.busy = true;
+  if (e != hipSuccess) {
+    // report the failure to every request of the batch
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& r : s.reqs) {
+      RequestResult rr;
+      rr.status = -2;
+      rr.n_images = r.n;
+      results_[r.id] = std::move(rr);
+      free_slots(r.slot, r.n);
+    }
+    s.busy = false;
+    cv_done_.notify_all();
+    cv_space_.notify_all();
+  }
+  return true;
+}
+
+void Server::harvest(Slot& s) {
+  BatchContext& c = *s.ctx;
+  const uint32_t n = s.n_images;
+  hipStream_t st = c.stream();
+  hipError_t e = hipMemcpyAsync(s.h_count, c.count(), 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s.h_hashes, c.hashes(), (size_t)n * 256, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s.h_quality, c.quality(), (size_t)n * 32, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s.h_threshold, c.threshold_out(), (size_t)n * 32, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  uint64_t total = e == hipSuccess ? *s.h_count : 0;
+  std::vector<MatchHit> overflow;  // hits re-matched when the batch buffer was too small
+  const MatchHit* hits = s.h_hits;
+  if (e == hipSuccess && total > c.hit_capacity()) {
+    // rerun the match of this batch into a buffer that holds every hit
+    MatchHit* d_big = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    e = hipMalloc(&d_big, total * sizeof(MatchHit));
+    if (e == hipSuccess) e = hipMalloc(&d_cnt, 8);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, 8, st);
+    if (e == hipSuccess)
+      e = shards_->launch_match(c.choice_for(n * 8), c.hashes(), nullptr, n * 8, cfg_.threshold, d_big, total, d_cnt,
+                                st);
+    if (e == hipSuccess) {
+      overflow.resize(total);
+      e = hipMemcpyAsync(overflow.data(), d_big, total * sizeof(MatchHit), hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_big) (void)hipFree(d_big);
+    if (d_cnt) (void)hipFree(d_cnt);
+    hits = overflow.data();
+  } else if (e == hipSuccess && total) {
+    e = hipMemcpyAsync(s.h_hits, c.hits(), total * sizeof(MatchHit), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  const auto t_done = std::chrono::steady_clock::now();
+  std::vector<RequestResult> rr(s.reqs.size());
+  for (size_t i = 0; i < s.reqs.size(); i++) {
+    const uint32_t off = s.offsets[i], m = s.reqs[i].n;
+    RequestResult& r = rr[i];
+    r.status = e == hipSuccess ? 0 : -2;
+    r.n_images = m;
+    r.hashes.assign(s.h_hashes + (size_t)off * 256, s.h_hashes + (size_t)(off + m) * 256);
+    r.quality.assign(s.h_quality + (size_t)off * 8, s.h_quality + (size_t)(off + m) * 8);
+    r.threshold.assign(s.h_threshold + (size_t)off * 8, s.h_threshold + (size_t)(off + m) * 8);
+    r.latency_ms = std::chrono::duration<double, std::milli>(t_done - s.reqs[i].t_submit).count();
+    r.batch = s.batch_no;
+    r.batch_images = n;
+  }
+  if (e == hipSuccess) {
+    for (uint64_t h = 0; h < total; h++) {
+      const uint32_t img = hits[h].query / 8;
+      // the request whose image range holds img
+      const size_t i = (size_t)(std::upper_bound(s.offsets.begin(), s.offsets.end(), img) - s.offsets.begin()) - 1;
+      MatchHit x = hits[h];
+      x.query -= 8 * s.offsets[i];
+      rr[i].hits.push_back(x);
+    }
+    for (auto& r : rr)
+      std::sort(r.hits.begin(), r.hits.end(), [](const MatchHit& a, const MatchHit& b) {
+        return a.index != b.index ? a.index < b.index : a.query < b.query;
+      });
+  }
+  images_ += n;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (size_t i = 0; i < s.reqs.size(); i++) {
+      free_slots(s.reqs[i].slot, s.reqs[i].n);
+      results_[s.reqs[i].id] = std::move(rr[i]);
+    }
+  }
+  s.busy = false;
+  cv_done_.notify_all();
+  cv_space_.notify_all();
+}
+
+void Server::loop() {
+  for (;;) {
+    bool progress = false, any_busy = false;
+    for (auto& sp : ctx_) {
+      Slot& s = *sp;
+      if (s.busy && hipEventQuery(s.done) == hipSuccess) {
+        harvest(s);
+        progress = true;
+      }
+      any_busy = any_busy || s.busy;
+    }
+    for (auto& sp : ctx_) {
+      if (sp->busy) continue;
+      if (launch_batch(*sp)) {
+        progress = true;
+        any_busy = true;
+      }
+    }
+    if (progress) continue;
+    std::unique_lock<std::mutex> g(mu_);
+    if (stop_ && pending_.empty() && !any_busy) break;
+    if (!pending_.empty() && !any_busy) continue;  // a context is free: go launch
+    if (any_busy)
+      cv_submit_.wait_for(g, std::chrono::microseconds(20));  // poll the in-flight batches
+    else
+      cv_submit_.wait_for(g, std::chrono::milliseconds(5), [&] { return stop_ || !pending_.empty(); });
+  }
+}
+
+}  // namespace ypdq

@@ -48,6 +48,23 @@ class BatchBuffers(C.Structure):
                 ("max_images", C.c_uint32), ("device_bytes", C.c_uint64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch_images", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("max_contexts", C.c_uint32),
+                ("threshold", C.c_uint32), ("dihedral_mode", C.c_uint32), ("hit_capacity", C.c_uint64),
+                ("use_graph", C.c_int)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("n_images", C.c_uint32), ("hashes", C.c_void_p), ("quality", C.c_void_p),
+                ("threshold", C.c_void_p), ("hits", C.c_void_p), ("n_hits", C.c_uint64),
+                ("latency_ms", C.c_double), ("batch", C.c_uint64), ("batch_images", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("contexts", C.c_uint32), ("device_bytes", C.c_uint64), ("batches", C.c_uint64),
+                ("images", C.c_uint64)]
+
+
 class TuneRecord(C.Structure):
     _fields_ = [("n_queries", C.c_uint32), ("path", C.c_uint32), ("query_tiles", C.c_uint32), ("ms", C.c_float)]
 
@@ -102,7 +119,13 @@ def lib():
                 ("yume_pdq_amd_batcher_autotune", i, [vp, i]),
                 ("yume_pdq_amd_batcher_tune_records", i, [vp, C.POINTER(TuneRecord), u32,
                                                           C.POINTER(C.c_uint32)]),
-                ("yume_pdq_amd_batcher_choice", i, [vp, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)])):
+                ("yume_pdq_amd_batcher_choice", i, [vp, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+                ("yume_pdq_amd_server_create", i, [vp, C.POINTER(ServerConfig), pp]),
+                ("yume_pdq_amd_server_destroy", None, [vp]),
+                ("yume_pdq_amd_server_submit", i, [vp, vp, u32, C.POINTER(C.c_uint64)]),
+                ("yume_pdq_amd_server_wait", i, [vp, u64, i, C.POINTER(ServerResult)]),
+                ("yume_pdq_amd_server_release", i, [vp, u64]),
+                ("yume_pdq_amd_server_stats", i, [vp, C.POINTER(ServerStats)])):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -624,3 +647,68 @@ class Batcher:
         _check(lib().yume_pdq_amd_batcher_choice(self._h, int(n_queries), C.byref(path), C.byref(qt)),
                "batcher_choice")
         return ("mfma" if path.value == MATCH_MFMA else "valu"), qt.value
+
+
+class Server:
+    """Continuous-batching server over a shard set: submit() u8 planes from
+    any thread, wait() for each request's hashes, qualities, thresholds and
+    hits (queries numbered 8 * image + dihedral within the request)."""
+
+    def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
+                 dihedral_mode=DIHEDRAL_EXACT, hit_capacity=1 << 20, use_graph=True):
+        cfg = ServerConfig(int(max_batch_images), int(hbm_budget_bytes), int(max_contexts), int(threshold),
+                           int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0)
+        h = C.c_void_p()
+        _check(lib().yume_pdq_amd_server_create(shards._h, C.byref(cfg), C.byref(h)), "server_create")
+        self._h = h
+        self._shards = shards
+        self.max_batch_images = int(max_batch_images)
+
+    def close(self):
+        if self._h:
+            lib().yume_pdq_amd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, planes):
+        p = np.ascontiguousarray(planes, np.uint8).reshape(-1, 512, 512)
+        rid = C.c_uint64(0)
+        _check(lib().yume_pdq_amd_server_submit(self._h, p.ctypes.data_as(C.c_void_p), p.shape[0], C.byref(rid)),
+               "server_submit")
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        r = ServerResult()
+        _check(lib().yume_pdq_amd_server_wait(self._h, int(rid), int(timeout_ms), C.byref(r)), "server_wait")
+        try:
+            n = r.n_images
+
+            def arr(ptr, count, dtype):
+                if count == 0:
+                    return np.zeros(0, dtype)
+                buf = (C.c_uint8 * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+                return np.frombuffer(buf, dtype=dtype).copy()
+
+            out = {
+                "status": r.status,
+                "hashes": arr(r.hashes, n * 256, np.uint8).reshape(n, 8, 32),
+                "quality": arr(r.quality, n * 8, np.float32).reshape(n, 8),
+                "threshold": arr(r.threshold, n * 8, np.float32).reshape(n, 8),
+                "hits": arr(r.hits, int(r.n_hits), HIT_DTYPE) if r.n_hits else np.zeros(0, HIT_DTYPE),
+                "latency_ms": r.latency_ms,
+                "batch": r.batch,
+                "batch_images": r.batch_images,
+            }
+        finally:
+            lib().yume_pdq_amd_server_release(self._h, int(rid))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().yume_pdq_amd_server_stats(self._h, C.byref(st)), "server_stats")
+        return {"contexts": st.contexts, "device_bytes": st.device_bytes, "batches": st.batches, "images": st.images}
