@@ -57,6 +57,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-images", type=int, default=8192)
     ap.add_argument("--verify", action="store_true", help="check B=1 hits against numpy (slow)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sections")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batching section")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-offered-10m", type=float, default=60_000.0,
+                    help="offered images/s of the 10M-row serving run")
+    ap.add_argument("--serve-offered-100m", type=float, default=8_000.0,
+                    help="offered images/s of the 100M-row serving run")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -269,6 +276,25 @@ def main():
     log("[bench] drop-in FFI: yume_pdq_hash_smart_kernel on a host f32 image")
     ffi_us = drop_in_ffi_us(yume_pdq, planes_u8[0].to(torch.float32).cpu().numpy())
 
+    pool_u8 = planes_u8[:256].cpu().numpy()
+    pool_hashes = h_all[:256].reshape(-1, 32).cpu().numpy()
+    serving = None
+    if not args.no_serving:
+        serving = [
+            serving_section(args, yume_pdq, torch, dev, synthetic, d_db, None, pool_u8, thr,
+                            offered=off, label=f"10M rows (the headline database), offered {off:.0f} images/s",
+                            max_contexts=3)
+            for off in (args.serve_offered_10m / 2, args.serve_offered_10m, 2.5 * args.serve_offered_10m)
+        ]
+        del d_plane
+        torch.cuda.empty_cache()
+        serving.append(serving_section(args, yume_pdq, torch, dev, synthetic, None, 100_000_000, pool_u8, thr,
+                                       offered=args.serve_offered_100m, pool_hashes=pool_hashes,
+                                       label="100M rows, 0.01 % planted"))
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_section(args, yume_pdq, torch, dev, synthetic, planes_u8[:256], pool_hashes, thr)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, db, thr, n_rows)
@@ -312,6 +338,8 @@ def main():
         "plain_unbatched_f32": plain,
         "speedup_vs_plain_unbatched": round(head["images_per_s"] / plain["images_per_s"], 2),
         "cpu_baseline": cpu,
+        "serving": serving,
+        "uneven_batching": uneven,
         "drop_in_ffi": {
             "us_per_hash": round(ffi_us, 2),
             "note": ("PCIe-inclusive per-image rate of the reference ABI: 1 MiB f32 host image "
@@ -320,6 +348,179 @@ def main():
         },
     }
     print(json.dumps(line), flush=True)
+
+
+def plain_host_rate(yume_pdq, torch, dev, d_db, pool_u8, thr, n=60):
+    """The plain unbatched GPU path from host memory: per image, upload the u8
+    plane (from the same page-locked pool), hash it (one launch), match its 8
+    hashes against the raw rows, read the hit count.  Returns images/s."""
+    hits = torch.empty((1 << 16, 16), dtype=torch.uint8, device=dev)
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    for i in range(5):
+        h, _, _ = yume_pdq.hash_batch(pool_u8[i:i + 1].to(dev))
+        yume_pdq.match(d_db, h.reshape(-1, 32), thr, hits=hits, count=count, sync=False)
+        count.item()
+    t0 = time.perf_counter()
+    for i in range(n):
+        h, _, _ = yume_pdq.hash_batch(pool_u8[i % pool_u8.shape[0]][None].to(dev))
+        yume_pdq.match(d_db, h.reshape(-1, 32), thr, hits=hits, count=count, sync=False)
+        count.item()
+    return n / (time.perf_counter() - t0)
+
+
+def serving_section(args, yume_pdq, torch, dev, synthetic, d_db, db_rows, pool_u8, thr, offered, label,
+                    pool_hashes=None, clients=8, max_batch=64, max_contexts=4):
+    """Open-loop Poisson arrivals at `offered` images/s of requests carrying
+    U{1..64} u8 512x512 images from host memory, served by the continuous-
+    batching server (hash + 8-dihedral match against every shard); latency is
+    submit -> results on the host, PCIe included."""
+    import threading
+
+    import numpy as np
+    if d_db is None:
+        log(f"[serving] building {db_rows} rows, 0.01 % planted")
+        db = synthetic.random_db(db_rows, seed=17)
+        synthetic.plant_fast(db, pool_hashes, max(1, db_rows // 10_000), seed=18)
+        d_db = torch.from_numpy(db).to(dev)
+        del db
+    n_rows = d_db.shape[0]
+    shards = yume_pdq.Shards()
+    shards.add(d_db)  # the rows stay where they are; the set adds the prefix plane
+    srv = yume_pdq.Server(shards, max_batch_images=max_batch, max_contexts=max_contexts, hbm_budget_bytes=16 << 30,
+                          threshold=thr)
+    rng = np.random.default_rng(23)
+    qps = offered / 32.5
+    gaps = rng.exponential(1.0 / qps, size=int(qps * args.serve_seconds * 1.2) + 10)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < args.serve_seconds]
+    sizes = rng.integers(1, 65, size=arrivals.size)
+    starts = rng.integers(0, pool_u8.shape[0], size=arrivals.size)
+    # request bodies sit in page-locked host memory (as a network stack's
+    # receive buffers would) and are submitted without a copy
+    pool2 = torch.from_numpy(np.concatenate([pool_u8, pool_u8])).pin_memory()
+    log(f"[serving] {label}: warm-up")
+    for j in range(8):
+        srv.wait(srv.submit_pinned(pool2[j:j + 64]))
+    log(f"[serving] {label}: {arrivals.size} requests over {args.serve_seconds} s "
+        f"(offered {offered:.0f} images/s), {clients} client threads")
+    ids = [0] * arrivals.size
+    lag = np.zeros(arrivals.size)  # submit later than the scheduled arrival (client behind)
+    t0 = time.perf_counter() + 0.05
+
+    def client(k):
+        for j in range(k, arrivals.size, clients):
+            delay = t0 + arrivals[j] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            lag[j] = max(0.0, time.perf_counter() - (t0 + arrivals[j]))
+            ids[j] = srv.submit_pinned(pool2[starts[j]:starts[j] + sizes[j]])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(clients)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    res = [srv.wait(i) for i in ids]
+    # open-loop latency: from the scheduled arrival, not from the actual submit
+    lat = np.array([r["latency_ms"] for r in res]) + lag * 1e3
+    done = np.array([arrivals[j] + lat[j] / 1e3 for j in range(arrivals.size)])
+    images = int(sizes.sum())
+    span = max(float(done.max()), float(arrivals.max()))
+    st = srv.stats()
+    bad = sum(1 for r in res if r["status"] != 0)
+    srv.close()
+    shards.close()
+    log(f"[serving] {label}: plain unbatched path from host memory")
+    plain = plain_host_rate(yume_pdq, torch, dev, d_db, pool2, thr)
+    served = images / span
+    out = {
+        "database": label,
+        "db_rows": int(n_rows),
+        "requests": int(arrivals.size),
+        "images": images,
+        "offered_images_per_s": round(offered, 1),
+        "served_images_per_s": round(served, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+        "batches": int(st["batches"]),
+        "mean_batch_images": round(float(np.mean([r["batch_images"] for r in res])), 1),
+        "contexts": st["contexts"],
+        "errors": bad,
+        "plain_unbatched_host_images_per_s": round(plain, 1),
+        "served_vs_plain": round(served / plain, 2),
+        "client_lag_ms_p99": round(float(np.percentile(lag, 99) * 1e3), 3),
+        "note": ("latency from the scheduled arrival to results on the host; request planes in "
+                 "page-locked host memory, uploaded by the server (PCIe included); plain = one image per "
+                 "call from the same memory: upload, hash launch, 8-query scan of the raw rows, count read"),
+    }
+    log(f"[serving]   {out}")
+    return out
+
+
+def uneven_section(args, yume_pdq, torch, dev, synthetic, planes_u8, pool_hashes, thr, n_batches=40):
+    """Uneven batches of U{1..256} images hashed and matched against 6 shards
+    of 10k, 100k, 1M, 5M, 20M and 50M rows (1 % planted) by the batch
+    context: one graph replay per batch, matcher per batch size chosen by the
+    autotuner."""
+    import numpy as np
+    sizes = [10_000, 100_000, 1_000_000, 5_000_000, 20_000_000, 50_000_000]
+    shards = yume_pdq.Shards()
+    keep = []
+    for k, n in enumerate(sizes):
+        log(f"[uneven] shard {k}: {n} rows, 1 % planted")
+        db = synthetic.random_db(n, seed=300 + k)
+        synthetic.plant_fast(db, pool_hashes, max(1, n // 100), seed=400 + k)
+        d = torch.from_numpy(db).to(dev)
+        del db
+        keep.append(d)
+        shards.add(d)
+    b = yume_pdq.Batcher(shards, 256, threshold=thr, hit_capacity=1 << 22)
+    log("[uneven] autotuning the matcher per batch size")
+    tune = b.autotune(iterations=3)
+    choices = {}
+    for r in tune:
+        best = min((x for x in tune if x["n_queries"] == r["n_queries"]), key=lambda x: x["ms"])
+        choices[r["n_queries"]] = f"{best['path']}" + (f" x{best['query_tiles']}" if best["path"] == "mfma" else "")
+    rng = np.random.default_rng(29)
+    ns = rng.integers(1, 257, size=n_batches)
+
+    def one(n):
+        # planes in, one graph replay, the hit count back on the host (the
+        # hashes and hits stay in the context's device buffers)
+        b.run(planes_u8[:n], sync=False)
+        with torch.cuda.stream(b.stream):
+            c = b.count.clone()
+        b.stream.synchronize()
+        return int(c.item())
+
+    for n in ns[:3]:
+        one(n)
+    ms, hits = [], []
+    t0 = time.perf_counter()
+    for n in ns:
+        s0 = time.perf_counter()
+        hits.append(one(n))
+        ms.append((time.perf_counter() - s0) * 1e3)
+    total = time.perf_counter() - t0
+    b.close()
+    shards.close()
+    del keep
+    torch.cuda.empty_cache()
+    out = {
+        "shards": sizes,
+        "batches": int(n_batches),
+        "images": int(ns.sum()),
+        "images_per_s": round(float(ns.sum()) / total, 1),
+        "batch_ms": {"mean": round(float(np.mean(ms)), 3), "p50": round(float(np.percentile(ms, 50)), 3),
+                     "p99": round(float(np.percentile(ms, 99)), 3)},
+        "autotuned_matcher_by_queries": choices,
+        "mean_hits_per_batch": round(float(np.mean(hits)), 1),
+        "note": ("per batch: planes copied into the context (device to device), one graph replay, the hit "
+                 "count read back; queries are hashes of the images the 1 % plants were made from"),
+    }
+    log(f"[uneven]   {out}")
+    return out
 
 
 def drop_in_ffi_us(yume_pdq, image, n=300):

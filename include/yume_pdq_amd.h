@@ -294,6 +294,7 @@ typedef struct YumePdqAmdServerConfig {
   uint32_t dihedral_mode;
   uint64_t hit_capacity;      /* per batch (a larger batch result is re-matched) */
   int use_graph;
+  int busy_poll;              /* worker spins (yielding) while batches are in flight */
 } YumePdqAmdServerConfig;
 typedef struct YumePdqAmdResult {
   int status;                 /* YUME_PDQ_AMD_OK or an error of the batch */
@@ -318,6 +319,10 @@ int yume_pdq_amd_server_create(YumePdqAmdShards *shards, const YumePdqAmdServerC
 void yume_pdq_amd_server_destroy(YumePdqAmdServer *server);
 int yume_pdq_amd_server_submit(YumePdqAmdServer *server, const uint8_t *planes, uint32_t n_images,
                                uint64_t *request_id);
+/* Zero-copy submit: `planes` is page-locked host memory (hipHostMalloc)
+ * that the caller leaves unchanged until the request's result is ready. */
+int yume_pdq_amd_server_submit_pinned(YumePdqAmdServer *server, const uint8_t *planes,
+                                      uint32_t n_images, uint64_t *request_id);
 /* Result pointers stay valid until yume_pdq_amd_server_release(id). */
 int yume_pdq_amd_server_wait(YumePdqAmdServer *server, uint64_t request_id, int timeout_ms,
                              YumePdqAmdResult *out);

@@ -114,6 +114,29 @@ def test_alone_equals_mix(gpu, dev, pool, served_db):
         srv.close()
 
 
+def test_pinned_submit_equals_staged(gpu, dev, pool, served_db):
+    """Zero-copy submission from page-locked memory gives the same results."""
+    import torch
+    shards, _ = served_db
+    pinned = torch.from_numpy(pool).pin_memory()
+    srv = gpu.Server(shards, max_batch_images=64, max_contexts=2)
+    try:
+        say("[server] pinned submit")
+        reqs = requests(12, seed=3)
+        ids = [srv.submit(pool[r]) for r in reqs]
+        # contiguous slices of the pinned pool, as the pinned form requires
+        starts = [int(r[0]) % 32 for r in reqs]
+        ids_p = [srv.submit_pinned(pinned[s0:s0 + len(r)]) for s0, r in zip(starts, reqs)]
+        for r, s0, i, j in zip(reqs, starts, ids, ids_p):
+            a = srv.wait(i)
+            b = srv.wait(j)
+            c = srv.wait(srv.submit(pool[s0:s0 + len(r)]))
+            assert same(b, c)
+            assert a["status"] == 0
+    finally:
+        srv.close()
+
+
 def test_one_context_budget(gpu, dev, pool, served_db):
     """A budget that holds one batch context serves one batch at a time,
     with the same results."""

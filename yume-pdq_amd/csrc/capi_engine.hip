@@ -192,6 +192,7 @@ int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig
   c.dihedral_mode = cfg->dihedral_mode;
   c.hit_capacity = cfg->hit_capacity;
   c.use_graph = cfg->use_graph != 0;
+  c.busy_poll = cfg->busy_poll != 0;
   const hipError_t e = v->server.start(s->set, c);
   if (e != hipSuccess) {
     delete v;
@@ -206,6 +207,12 @@ void yume_pdq_amd_server_destroy(YumePdqAmdServer* v) { delete v; }
 int yume_pdq_amd_server_submit(YumePdqAmdServer* v, const uint8_t* planes, uint32_t n_images, uint64_t* id) {
   if (!v || !planes || !id) return YUME_PDQ_AMD_EINVAL;
   *id = v->server.submit(planes, n_images);
+  return *id ? YUME_PDQ_AMD_OK : YUME_PDQ_AMD_EINVAL;
+}
+
+int yume_pdq_amd_server_submit_pinned(YumePdqAmdServer* v, const uint8_t* planes, uint32_t n_images, uint64_t* id) {
+  if (!v || !planes || !id) return YUME_PDQ_AMD_EINVAL;
+  *id = v->server.submit_pinned(planes, n_images);
   return *id ? YUME_PDQ_AMD_OK : YUME_PDQ_AMD_EINVAL;
 }
 

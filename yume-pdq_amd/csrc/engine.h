@@ -109,6 +109,13 @@ class BatchContext {
   uint32_t max_images() const { return max_images_; }
   uint64_t hit_capacity() const { return hit_capacity_; }
   uint64_t device_bytes() const;
+  // The outputs live in one device block -- hit count, hashes, qualities,
+  // thresholds, hits, each at out_offset() -- so that one copy can read a
+  // batch back: out_bytes(k) covers everything up to the first k hits.
+  const void* out_block() const { return d_out_; }
+  enum OutPart { kOutCount = 0, kOutHashes, kOutQuality, kOutThreshold, kOutHits };
+  uint64_t out_offset(OutPart p) const;
+  uint64_t out_bytes(uint64_t hits_prefix) const { return out_offset(kOutHits) + hits_prefix * sizeof(MatchHit); }
 
  private:
   struct Graph {
@@ -128,6 +135,7 @@ class BatchContext {
   uint64_t plane_stride_ = 0;
   hipStream_t stream_ = nullptr;
   void* d_input_ = nullptr;
+  uint8_t* d_out_ = nullptr;  // the output block (see out_offset)
   uint8_t* d_hashes_ = nullptr;
   float* d_quality_ = nullptr;
   float* d_threshold_ = nullptr;

@@ -210,7 +210,7 @@ BatchContext::~BatchContext() {
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
   }
-  void* bufs[] = {d_input_, d_hashes_, d_quality_, d_threshold_, d_hits_, d_count_, d_sizes_};
+  void* bufs[] = {d_input_, d_out_, d_sizes_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h_sizes_) (void)hipHostFree(h_sizes_);
@@ -232,11 +232,12 @@ hipError_t BatchContext::init(std::shared_ptr<ShardSet> shards, uint32_t max_ima
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipMalloc(&d_input_, plane_stride_ * max_images)) != hipSuccess) return e;
-  if ((e = hipMalloc(&d_hashes_, (size_t)max_images * 256)) != hipSuccess) return e;
-  if ((e = hipMalloc(&d_quality_, (size_t)max_images * 32)) != hipSuccess) return e;
-  if ((e = hipMalloc(&d_threshold_, (size_t)max_images * 32)) != hipSuccess) return e;
-  if ((e = hipMalloc(&d_hits_, hit_capacity * sizeof(MatchHit))) != hipSuccess) return e;
-  if ((e = hipMalloc(&d_count_, sizeof(unsigned long long))) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_out_, out_bytes(hit_capacity))) != hipSuccess) return e;
+  d_count_ = reinterpret_cast<unsigned long long*>(d_out_ + out_offset(kOutCount));
+  d_hashes_ = d_out_ + out_offset(kOutHashes);
+  d_quality_ = reinterpret_cast<float*>(d_out_ + out_offset(kOutQuality));
+  d_threshold_ = reinterpret_cast<float*>(d_out_ + out_offset(kOutThreshold));
+  d_hits_ = d_out_ + out_offset(kOutHits);
   if ((e = hipMalloc(&d_sizes_, 2 * sizeof(uint32_t))) != hipSuccess) return e;
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_sizes_), 2 * sizeof(uint32_t), hipHostMallocDefault)) !=
       hipSuccess)
@@ -247,9 +248,18 @@ hipError_t BatchContext::init(std::shared_ptr<ShardSet> shards, uint32_t max_ima
   return hipSuccess;
 }
 
-uint64_t BatchContext::device_bytes() const {
-  return plane_stride_ * max_images_ + (uint64_t)max_images_ * (256 + 64) + hit_capacity_ * sizeof(MatchHit) + 16;
+uint64_t BatchContext::out_offset(OutPart p) const {
+  const uint64_t m = max_images_;
+  switch (p) {
+    case kOutCount: return 0;
+    case kOutHashes: return 256;
+    case kOutQuality: return 256 + m * 256;
+    case kOutThreshold: return 256 + m * 256 + m * 32;
+    default: return (256 + m * 256 + m * 64 + 255) / 256 * 256;
+  }
 }
+
+uint64_t BatchContext::device_bytes() const { return plane_stride_ * max_images_ + out_bytes(hit_capacity_) + 16; }
 
 MatcherChoice BatchContext::choice_for(uint32_t nq) const {
   auto it = choices_.upper_bound(nq);

@@ -27,6 +27,7 @@ struct ServerConfig {
   uint32_t dihedral_mode = kDihedralExact;
   uint64_t hit_capacity = 1 << 20;  // per batch; a batch with more hits is re-matched into a larger buffer
   bool use_graph = true;
+  bool busy_poll = true;  // spin (yielding) while batches are in flight instead of sleeping between polls
 };
 
 struct RequestResult {
@@ -53,6 +54,9 @@ class Server {
   // Copies n u8 planes (n x 512 x 512) into pinned staging; blocks while the
   // staging ring is full.  Returns the request id (> 0) or 0 on bad input.
   uint64_t submit(const uint8_t* planes, uint32_t n);
+  // Zero-copy form: the planes already sit in page-locked host memory that
+  // the caller keeps unchanged until the request's result is ready.
+  uint64_t submit_pinned(const uint8_t* planes, uint32_t n);
   // Waits up to timeout_ms (< 0: forever) for a request; true when done.
   bool wait(uint64_t id, int timeout_ms, RequestResult* out);
   uint32_t contexts() const { return (uint32_t)ctx_.size(); }
@@ -64,7 +68,9 @@ class Server {
   struct Request {
     uint64_t id;
     uint32_t n;
-    uint64_t slot;  // first staging slot
+    uint64_t slot;       // first staging slot (staged requests)
+    const uint8_t* src;  // where the planes are read from (staging or the caller's pinned memory)
+    bool staged;
     std::chrono::steady_clock::time_point t_submit;
   };
   struct Slot {  // one batch context and the requests it is running
@@ -75,13 +81,10 @@ class Server {
     uint32_t n_images = 0;
     std::vector<Request> reqs;
     std::vector<uint32_t> offsets;  // first image of each request in the batch
-    // pinned readback staging
-    uint8_t* h_hashes = nullptr;
-    float* h_quality = nullptr;
-    float* h_threshold = nullptr;
-    unsigned long long* h_count = nullptr;
+    // pinned readback: the context's output block up to the first
+    // kHitPrefix hits (one copy per batch), and the rest of the hits
+    uint8_t* h_out = nullptr;
     MatchHit* h_hits = nullptr;
-    uint64_t h_hits_cap = 0;
   };
   void loop();
   bool launch_batch(Slot& s);

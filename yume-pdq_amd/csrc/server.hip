@@ -21,6 +21,12 @@
 
 namespace ypdq {
 
+namespace {
+// Hits read back with the batch's hashes in one copy; a batch with more takes
+// a second copy for the rest.
+constexpr uint64_t kHitPrefix = 4096;
+}  // namespace
+
 Server::~Server() { stop(); }
 
 uint64_t Server::device_bytes() const {
@@ -46,19 +52,18 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
     // admission by HBM budget: a context is kept only if it fits
     if (device_bytes() + s->ctx->device_bytes() > cfg.hbm_budget_bytes) break;
     if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;
-    const uint32_t m = cfg.max_batch_images;
-    if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_hashes), (size_t)m * 256, hipHostMallocDefault)) ||
-        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_quality), (size_t)m * 32, hipHostMallocDefault)) ||
-        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_threshold), (size_t)m * 32, hipHostMallocDefault)) ||
-        (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_count), 8, hipHostMallocDefault)) ||
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out),
+                           s->ctx->out_bytes(std::min<uint64_t>(kHitPrefix, cfg.hit_capacity)),
+                           hipHostMallocDefault)) ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_hits), cfg.hit_capacity * sizeof(MatchHit),
                            hipHostMallocDefault)))
       return e;
-    s->h_hits_cap = cfg.hit_capacity;
     ctx_.push_back(std::move(s));
   }
   if (ctx_.empty()) return hipErrorOutOfMemory;  // the budget does not hold one batch
-  ring_slots_ = (uint64_t)(ctx_.size() + 1) * cfg.max_batch_images;
+  // room for every context's batch plus as much again queued, so that
+  // requests rarely wait for contiguous staging
+  ring_slots_ = (uint64_t)(2 * ctx_.size() + 2) * cfg.max_batch_images;
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&staging_), ring_slots_ * slot_bytes_, hipHostMallocDefault)) !=
       hipSuccess)
     return e;
@@ -77,7 +82,7 @@ void Server::stop() {
   if (worker_.joinable()) worker_.join();
   for (auto& s : ctx_) {
     if (s->done) (void)hipEventDestroy(s->done);
-    void* h[] = {s->h_hashes, s->h_quality, s->h_threshold, s->h_count, s->h_hits};
+    void* h[] = {s->h_out, s->h_hits};
     for (void* p : h)
       if (p) (void)hipHostFree(p);
   }
@@ -107,7 +112,21 @@ uint64_t Server::submit(const uint8_t* planes, uint32_t n) {
   memcpy(staging_ + first * slot_bytes_, planes, (size_t)n * slot_bytes_);
   {
     std::lock_guard<std::mutex> g(mu_);
-    pending_.push_back(Request{id, n, first, t0});
+    pending_.push_back(Request{id, n, first, staging_ + first * slot_bytes_, true, t0});
+  }
+  cv_submit_.notify_one();
+  return id;
+}
+
+uint64_t Server::submit_pinned(const uint8_t* planes, uint32_t n) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!planes || n == 0 || n > cfg_.max_batch_images || !worker_.joinable()) return 0;
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    if (stop_) return 0;
+    id = next_id_++;
+    pending_.push_back(Request{id, n, 0, planes, false, t0});
   }
   cv_submit_.notify_one();
   return id;
@@ -146,12 +165,21 @@ bool Server::launch_batch(Slot& s) {
   if (total == 0) return false;
   BatchContext& c = *s.ctx;
   hipError_t e = hipSuccess;
-  for (size_t i = 0; i < s.reqs.size() && e == hipSuccess; i++)
-    e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.offsets[i] * slot_bytes_,
-                       staging_ + s.reqs[i].slot * slot_bytes_, (size_t)s.reqs[i].n * slot_bytes_,
+  // one upload per run of requests whose planes are adjacent in host memory
+  for (size_t i = 0; i < s.reqs.size() && e == hipSuccess;) {
+    size_t j = i + 1;
+    uint64_t bytes = (uint64_t)s.reqs[i].n * slot_bytes_;
+    while (j < s.reqs.size() && s.reqs[j].src == s.reqs[i].src + bytes) bytes += (uint64_t)s.reqs[j++].n * slot_bytes_;
+    e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.offsets[i] * slot_bytes_, s.reqs[i].src, bytes,
                        hipMemcpyHostToDevice, c.stream());
+    i = j;
+  }
   // the context's previous batch was harvested, so there is nothing to wait for
   if (e == hipSuccess) e = c.run(total, cfg_.use_graph, /*wait_previous=*/false);
+  // the outputs and the first hits come back in the same stream order
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s.h_out, c.out_block(), c.out_bytes(std::min<uint64_t>(kHitPrefix, c.hit_capacity())),
+                       hipMemcpyDeviceToHost, c.stream());
   if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
   s.n_images = total;
   s.batch_no = batches_.fetch_add(1) + 1;
@@ -164,7 +192,7 @@ bool Server::launch_batch(Slot& s) {
       rr.status = -2;
       rr.n_images = r.n;
       results_[r.id] = std::move(rr);
-      free_slots(r.slot, r.n);
+      if (r.staged) free_slots(r.slot, r.n);
     }
     s.busy = false;
     cv_done_.notify_all();
@@ -177,15 +205,22 @@ void Server::harvest(Slot& s) {
   BatchContext& c = *s.ctx;
   const uint32_t n = s.n_images;
   hipStream_t st = c.stream();
-  hipError_t e = hipMemcpyAsync(s.h_count, c.count(), 8, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(s.h_hashes, c.hashes(), (size_t)n * 256, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(s.h_quality, c.quality(), (size_t)n * 32, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(s.h_threshold, c.threshold_out(), (size_t)n * 32, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  uint64_t total = e == hipSuccess ? *s.h_count : 0;
+  hipError_t e = hipSuccess;
+  const uint8_t* h_hashes = s.h_out + c.out_offset(BatchContext::kOutHashes);
+  const float* h_quality = reinterpret_cast<const float*>(s.h_out + c.out_offset(BatchContext::kOutQuality));
+  const float* h_threshold = reinterpret_cast<const float*>(s.h_out + c.out_offset(BatchContext::kOutThreshold));
+  const uint64_t total = *reinterpret_cast<const unsigned long long*>(s.h_out + c.out_offset(BatchContext::kOutCount));
+  const uint64_t prefix = std::min<uint64_t>(kHitPrefix, c.hit_capacity());
   std::vector<MatchHit> overflow;  // hits re-matched when the batch buffer was too small
-  const MatchHit* hits = s.h_hits;
-  if (e == hipSuccess && total > c.hit_capacity()) {
+  const MatchHit* hits = reinterpret_cast<const MatchHit*>(s.h_out + c.out_offset(BatchContext::kOutHits));
+  if (total > prefix && total <= c.hit_capacity()) {
+    // the hits past the prefix
+    memcpy(s.h_hits, hits, prefix * sizeof(MatchHit));
+    e = hipMemcpyAsync(s.h_hits + prefix, static_cast<const MatchHit*>(c.hits()) + prefix,
+                       (total - prefix) * sizeof(MatchHit), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hits = s.h_hits;
+  } else if (total > c.hit_capacity()) {
     // rerun the match of this batch into a buffer that holds every hit
     MatchHit* d_big = nullptr;
     unsigned long long* d_cnt = nullptr;
@@ -203,9 +238,6 @@ void Server::harvest(Slot& s) {
     if (d_big) (void)hipFree(d_big);
     if (d_cnt) (void)hipFree(d_cnt);
     hits = overflow.data();
-  } else if (e == hipSuccess && total) {
-    e = hipMemcpyAsync(s.h_hits, c.hits(), total * sizeof(MatchHit), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
   const auto t_done = std::chrono::steady_clock::now();
   std::vector<RequestResult> rr(s.reqs.size());
@@ -214,9 +246,9 @@ void Server::harvest(Slot& s) {
     RequestResult& r = rr[i];
     r.status = e == hipSuccess ? 0 : -2;
     r.n_images = m;
-    r.hashes.assign(s.h_hashes + (size_t)off * 256, s.h_hashes + (size_t)(off + m) * 256);
-    r.quality.assign(s.h_quality + (size_t)off * 8, s.h_quality + (size_t)(off + m) * 8);
-    r.threshold.assign(s.h_threshold + (size_t)off * 8, s.h_threshold + (size_t)(off + m) * 8);
+    r.hashes.assign(h_hashes + (size_t)off * 256, h_hashes + (size_t)(off + m) * 256);
+    r.quality.assign(h_quality + (size_t)off * 8, h_quality + (size_t)(off + m) * 8);
+    r.threshold.assign(h_threshold + (size_t)off * 8, h_threshold + (size_t)(off + m) * 8);
     r.latency_ms = std::chrono::duration<double, std::milli>(t_done - s.reqs[i].t_submit).count();
     r.batch = s.batch_no;
     r.batch_images = n;
@@ -239,7 +271,7 @@ void Server::harvest(Slot& s) {
   {
     std::lock_guard<std::mutex> g(mu_);
     for (size_t i = 0; i < s.reqs.size(); i++) {
-      free_slots(s.reqs[i].slot, s.reqs[i].n);
+      if (s.reqs[i].staged) free_slots(s.reqs[i].slot, s.reqs[i].n);
       results_[s.reqs[i].id] = std::move(rr[i]);
     }
   }
@@ -270,8 +302,11 @@ void Server::loop() {
     std::unique_lock<std::mutex> g(mu_);
     if (stop_ && pending_.empty() && !any_busy) break;
     if (!pending_.empty() && !any_busy) continue;  // a context is free: go launch
-    if (any_busy)
-      cv_submit_.wait_for(g, std::chrono::microseconds(20));  // poll the in-flight batches
+    if (any_busy && cfg_.busy_poll) {
+      g.unlock();
+      std::this_thread::yield();  // poll the in-flight batches
+    } else if (any_busy)
+      cv_submit_.wait_for(g, std::chrono::microseconds(20));
     else
       cv_submit_.wait_for(g, std::chrono::milliseconds(5), [&] { return stop_ || !pending_.empty(); });
   }

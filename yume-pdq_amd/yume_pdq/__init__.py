@@ -51,7 +51,7 @@ class BatchBuffers(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch_images", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("max_contexts", C.c_uint32),
                 ("threshold", C.c_uint32), ("dihedral_mode", C.c_uint32), ("hit_capacity", C.c_uint64),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("busy_poll", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -123,6 +123,7 @@ def lib():
                 ("yume_pdq_amd_server_create", i, [vp, C.POINTER(ServerConfig), pp]),
                 ("yume_pdq_amd_server_destroy", None, [vp]),
                 ("yume_pdq_amd_server_submit", i, [vp, vp, u32, C.POINTER(C.c_uint64)]),
+                ("yume_pdq_amd_server_submit_pinned", i, [vp, vp, u32, C.POINTER(C.c_uint64)]),
                 ("yume_pdq_amd_server_wait", i, [vp, u64, i, C.POINTER(ServerResult)]),
                 ("yume_pdq_amd_server_release", i, [vp, u64]),
                 ("yume_pdq_amd_server_stats", i, [vp, C.POINTER(ServerStats)])):
@@ -655,9 +656,9 @@ class Server:
     hits (queries numbered 8 * image + dihedral within the request)."""
 
     def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
-                 dihedral_mode=DIHEDRAL_EXACT, hit_capacity=1 << 20, use_graph=True):
+                 dihedral_mode=DIHEDRAL_EXACT, hit_capacity=1 << 20, use_graph=True, busy_poll=True):
         cfg = ServerConfig(int(max_batch_images), int(hbm_budget_bytes), int(max_contexts), int(threshold),
-                           int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0)
+                           int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0, 1 if busy_poll else 0)
         h = C.c_void_p()
         _check(lib().yume_pdq_amd_server_create(shards._h, C.byref(cfg), C.byref(h)), "server_create")
         self._h = h
@@ -680,6 +681,17 @@ class Server:
         rid = C.c_uint64(0)
         _check(lib().yume_pdq_amd_server_submit(self._h, p.ctypes.data_as(C.c_void_p), p.shape[0], C.byref(rid)),
                "server_submit")
+        return rid.value
+
+    def submit_pinned(self, planes):
+        """Zero-copy submit of a pinned CPU torch tensor (n, 512, 512) u8 that
+        stays unchanged until the request's result has been waited for."""
+        if not planes.is_pinned() or not planes.is_contiguous() or planes.dtype != _torch().uint8:
+            raise ValueError("planes must be a contiguous pinned uint8 CPU tensor")
+        n = planes.numel() // (512 * 512)
+        rid = C.c_uint64(0)
+        _check(lib().yume_pdq_amd_server_submit_pinned(self._h, C.c_void_p(planes.data_ptr()), n, C.byref(rid)),
+               "server_submit_pinned")
         return rid.value
 
     def wait(self, rid, timeout_ms=-1):
