@@ -30,7 +30,17 @@ enum {
   YUME_PDQ_AMD_ETIMEOUT = -6   /* a wait ran out of time */
 };
 
-enum { YUME_PDQ_AMD_INPUT_F32 = 0, YUME_PDQ_AMD_INPUT_U8 = 1 };
+/* Input planes: f32 luma (0..255), u8 luma, or 8-bit colour converted to luma
+ * while the plane is read -- RGB8 as the Kernel trait's cvt_rgb8_to_luma8f
+ * (src/kernel/mod.rs:303-322), RGBA8 as the AVX2 kernel's
+ * cvt_rgba8_to_luma8f (src/kernel/x86.rs:686-753), BT.601 coefficients
+ * (src/kernel/constants.rs:23) unless given. */
+enum {
+  YUME_PDQ_AMD_INPUT_F32 = 0,
+  YUME_PDQ_AMD_INPUT_U8 = 1,
+  YUME_PDQ_AMD_INPUT_RGB8 = 2,
+  YUME_PDQ_AMD_INPUT_RGBA8 = 3
+};
 
 /* Dihedral sign algebra: EXACT negates every second column (the AVX-512
  * and scalar kernels, src/kernel/x86.rs:1026-1074, src/kernel/mod.rs:368-382);
@@ -72,9 +82,9 @@ int yume_pdq_amd_device_info(int device, char *name, size_t name_len, int *compu
  * Batched hash_get_threshold (src/lib.rs:427-455) + visit_dihedrals
  * (src/lib.rs:358-420), AVX2 arithmetic (src/kernel/x86.rs:100-682).
  *
- * d_planes: n_images planes of 512x512, f32 (values 0..255) or u8, plane i at
- *           byte offset i * plane_stride_bytes (>= 512*512*elem, 16-byte
- *           aligned).
+ * d_planes: n_images planes of 512x512 (f32 0..255, u8, RGB8 or RGBA8 pixels),
+ *           plane i at byte offset i * plane_stride_bytes (>= one plane,
+ *           a multiple of 16; d_planes 16-byte aligned).
  * d_hashes: n_images x 8 x 32 bytes, orientations in YUME_PDQ_AMD_D_* order.
  * d_quality, d_threshold: n_images x 8 floats (per-orientation values; the
  *           threshold of a transposed orientation is the one it was
@@ -84,6 +94,15 @@ int yume_pdq_amd_device_info(int device, char *name, size_t name_len, int *compu
 int yume_pdq_amd_hash_batch(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
                             uint32_t n_images, uint32_t dihedral_mode, uint8_t *d_hashes,
                             float *d_quality, float *d_threshold, float *d_pdqf, void *stream);
+
+/* yume_pdq_amd_hash_batch with the (R, G, B) luma coefficients of colour
+ * inputs given (luma_coeff[3]; NULL = BT.601).  Fuses cvt_*_to_luma8f into
+ * the hash: the result equals converting each row first and hashing the f32
+ * luma plane. */
+int yume_pdq_amd_hash_batch_luma(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
+                                 uint32_t n_images, uint32_t dihedral_mode, const float *luma_coeff,
+                                 uint8_t *d_hashes, float *d_quality, float *d_threshold, float *d_pdqf,
+                                 void *stream);
 
 /*
  * Batched visit_dihedrals (src/lib.rs:358-420) from given PDQFs.

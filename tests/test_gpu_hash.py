@@ -195,3 +195,28 @@ def test_dihedrals_batch(gpu, oracle, dev, imgs):
             assert same_f32(t[i, 1:], th)
             assert same_f32(pds[i, 1:], pd)
             assert same_f32(pds[i, 0], P[i])
+
+
+@pytest.mark.parametrize("channels", [3, 4])
+@pytest.mark.parametrize("coeff", [None, (0.2126, 0.7152, 0.0722)])
+def test_colour_inputs_fused(gpu, dev, channels, coeff):
+    """RGB8 / RGBA8 planes hashed with the luma conversion fused into the row
+    loader equal converting first (cvt_rgb8_to_luma8f / cvt_rgba8_to_luma8f,
+    checked against numpy in test_gpu_stages.test_luma) and hashing the f32
+    luma plane -- bit for bit, over an uneven batch."""
+    import torch
+    rng = np.random.default_rng(channels)
+    base = synthetic.images_u8(5).astype(np.int16)
+    px = np.empty((5, 512, 512, channels), np.uint8)
+    for c in range(channels):
+        px[..., c] = np.clip(base + rng.integers(-40, 41, size=(5, 1, 1)) +
+                             rng.integers(-6, 7, size=base.shape), 0, 255).astype(np.uint8)
+    d_px = torch.from_numpy(px).to(dev)
+    say(f"[hash] colour channels={channels} coeff={coeff}")
+    lum = gpu.stage_luma(d_px, coeff=coeff or gpu.RGB8_TO_LUMA8_TABLE_ITU)
+    for mode in (0, 1):
+        fh, fq, ft = gpu.hash_batch(d_px, dihedral_mode=mode, luma=coeff)
+        eh, eq, et = gpu.hash_batch(lum.contiguous(), dihedral_mode=mode)
+        assert torch.equal(fh, eh)
+        assert same_f32(fq.cpu().numpy(), eq.cpu().numpy())
+        assert same_f32(ft.cpu().numpy(), et.cpu().numpy())

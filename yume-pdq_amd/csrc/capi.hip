@@ -218,14 +218,36 @@ int yume_pdq_amd_device_info(int device, char* name, size_t name_len, int* compu
   return n;
 }
 
+int yume_pdq_amd_hash_batch_luma(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
+                                 uint32_t dihedral_mode, const float* luma_coeff, uint8_t* d_hashes, float* d_quality,
+                                 float* d_threshold, float* d_pdqf, void* stream) {
+  if (n_images == 0) return YUME_PDQ_AMD_OK;
+  if (!d_planes || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
+  if (plane_bytes == 0) return YUME_PDQ_AMD_EINVAL;
+  if (dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
+  if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
+    return YUME_PDQ_AMD_EINVAL;
+  HashOutputs o;
+  o.hashes = d_hashes;
+  o.quality = d_quality;
+  o.threshold = d_threshold;
+  o.pdqf = d_pdqf;
+  o.buf1 = nullptr;
+  o.tmp = nullptr;
+  o.dihedral_pdqf = nullptr;
+  return from_hip(ypdq::launch_hash(d_planes, input_kind, plane_stride_bytes, n_images, dihedral_mode, o,
+                                    static_cast<hipStream_t>(stream), luma_coeff));
+}
+
 int yume_pdq_amd_hash_batch(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
                             uint32_t dihedral_mode, uint8_t* d_hashes, float* d_quality, float* d_threshold,
                             float* d_pdqf, void* stream) {
   if (n_images == 0) return YUME_PDQ_AMD_OK;
   if (!d_planes || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
-  if (input_kind != YUME_PDQ_AMD_INPUT_F32 && input_kind != YUME_PDQ_AMD_INPUT_U8) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
+  if (plane_bytes == 0) return YUME_PDQ_AMD_EINVAL;
   if (dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
-  const uint64_t plane_bytes = 512ull * 512 * (input_kind == YUME_PDQ_AMD_INPUT_F32 ? 4 : 1);
   if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
     return YUME_PDQ_AMD_EINVAL;
   HashOutputs o;

@@ -9,6 +9,12 @@ constexpr uint32_t kDihedralExact = 0;  // sign algebra over all 16 columns
 constexpr uint32_t kDihedralAvx2 = 1;   // the AVX2 kernel's column extent (x86.rs:330-376)
 constexpr int kInputF32 = 0;
 constexpr int kInputU8 = 1;
+constexpr int kInputRgb8 = 2;   // 3 bytes per pixel, converted to luma while loading
+constexpr int kInputRgba8 = 3;  // 4 bytes per pixel
+// bytes of one 512 x 512 plane of an input kind (0 for an unknown kind)
+inline uint64_t input_plane_bytes(int kind) {
+  return 512ull * 512 * (kind == kInputF32 ? 4 : kind == kInputU8 ? 1 : kind == kInputRgb8 ? 3 : kind == kInputRgba8 ? 4 : 0);
+}
 
 // Per-image outputs of the hash kernels; every array is indexed by image.
 struct HashOutputs {
@@ -45,8 +51,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ bool wave_any(bool b) { return __any(b ? 1 : 0) != 0; }
 __device__ __forceinline__ bool wave_all(bool b) { return __all(b ? 1 : 0) != 0; }
 
+// luma: (R, G, B) coefficients of the colour inputs, null for BT.601.
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
-                       const HashOutputs& out, hipStream_t stream);
+                       const HashOutputs& out, hipStream_t stream, const float* luma = nullptr);
 // Grid for n_max images, image count read from d_n (capped at n_max).
 hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
                                     const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream);

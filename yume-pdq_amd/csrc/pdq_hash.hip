@@ -171,6 +171,14 @@ template <>
 struct Prefetch<uint8_t> {
   static constexpr int D = 4;
 };
+template <>
+struct Prefetch<Rgb8> {
+  static constexpr int D = 1;
+};
+template <>
+struct Prefetch<Rgba8> {
+  static constexpr int D = 1;
+};
 
 // One compressed row i (ring slot S = i mod D): rows 4i+3 .. 4i+6 arrive from
 // pre[S], each slot is refilled with the row D steps ahead as soon as it has
@@ -178,22 +186,23 @@ struct Prefetch<uint8_t> {
 // the two compressed values go into stage-1 accumulator P.
 template <int P, int S, int D, typename T, typename Raw>
 __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lane, int i, float (&acc)[16][2][2],
-                                            Chains& cur, Chains& nxt, Raw (&pre)[D][4], float* __restrict__ buf1) {
+                                            Chains& cur, Chains& nxt, Raw (&pre)[D][4], float* __restrict__ buf1,
+                                            const Luma& lc) {
   const bool more = i + D < 127;
   const int r0 = 4 * (i + D) + 3;
   Row8 x;
-  to_f32(pre[S][0], x);
+  to_f32(pre[S][0], x, lc);
   if (more) load_raw(plane, r0, lane, pre[S][0]);
   chains_feed<2>(cur, x);
-  to_f32(pre[S][1], x);
+  to_f32(pre[S][1], x, lc);
   if (more) load_raw(plane, r0 + 1, lane, pre[S][1]);
   chains_feed<3>(cur, x);
   chains_zero(nxt);
-  to_f32(pre[S][2], x);
+  to_f32(pre[S][2], x, lc);
   if (more) load_raw(plane, r0 + 2, lane, pre[S][2]);
   chains_feed<4>(cur, x);
   chains_feed<0>(nxt, x);
-  to_f32(pre[S][3], x);
+  to_f32(pre[S][3], x, lc);
   if (more) load_raw(plane, r0 + 3, lane, pre[S][3]);
   chains_feed<5>(cur, x);
   chains_feed<1>(nxt, x);
@@ -209,7 +218,7 @@ __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lan
 
 template <typename T>
 __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int lane, float (&acc)[16][2][2],
-                                             float* __restrict__ buf1) {
+                                             float* __restrict__ buf1, const Luma& lc) {
   using Raw = typename RawRow<T>::type;
   constexpr int D = Prefetch<T>::D;
   static_assert(4 % D == 0, "ring depth must divide the unroll factor");
@@ -230,9 +239,9 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
 #pragma unroll
       for (int d = 0; d < 4; d++) load_raw(plane, 4 * s + 3 + d, lane, pre[s][d]);
     Row8 x;
-    to_f32(r1, x);
+    to_f32(r1, x, lc);
     chains_feed<0>(cur, x);
-    to_f32(r2, x);
+    to_f32(r2, x, lc);
     chains_feed<1>(cur, x);
   }
   // The accumulator a compressed row feeds is fixed by i mod 4 (alt), so the
@@ -240,14 +249,14 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
   // constant.
 #pragma unroll 1
   for (int i = 0; i < 124; i += 4) {
-    stream_step<1, 0 % D, D>(plane, lane, i, acc, cur, nxt, pre, buf1);
-    stream_step<0, 1 % D, D>(plane, lane, i + 1, acc, cur, nxt, pre, buf1);
-    stream_step<0, 2 % D, D>(plane, lane, i + 2, acc, cur, nxt, pre, buf1);
-    stream_step<1, 3 % D, D>(plane, lane, i + 3, acc, cur, nxt, pre, buf1);
+    stream_step<1, 0 % D, D>(plane, lane, i, acc, cur, nxt, pre, buf1, lc);
+    stream_step<0, 1 % D, D>(plane, lane, i + 1, acc, cur, nxt, pre, buf1, lc);
+    stream_step<0, 2 % D, D>(plane, lane, i + 2, acc, cur, nxt, pre, buf1, lc);
+    stream_step<1, 3 % D, D>(plane, lane, i + 3, acc, cur, nxt, pre, buf1, lc);
   }
-  stream_step<1, 124 % D, D>(plane, lane, 124, acc, cur, nxt, pre, buf1);
-  stream_step<0, 125 % D, D>(plane, lane, 125, acc, cur, nxt, pre, buf1);
-  stream_step<0, 126 % D, D>(plane, lane, 126, acc, cur, nxt, pre, buf1);
+  stream_step<1, 124 % D, D>(plane, lane, 124, acc, cur, nxt, pre, buf1, lc);
+  stream_step<0, 125 % D, D>(plane, lane, 125, acc, cur, nxt, pre, buf1, lc);
+  stream_step<0, 126 % D, D>(plane, lane, 126, acc, cur, nxt, pre, buf1, lc);
 }
 
 // Stage-1 rows into LDS: tmp[k][j] = nan0(acc0) + nan0(acc1) (x86.rs:207-224);
@@ -269,11 +278,11 @@ __device__ __forceinline__ void write_stage1(const float (&acc)[16][2][2], WaveL
 
 template <int P, typename T>
 __device__ __forceinline__ void literal_step(const T* __restrict__ plane, int lane, int i, int k0,
-                                             float (&acc)[8][2][2], float* __restrict__ buf1) {
-  const float c0 = compress_one_literal(plane, i, 2 * lane);
+                                             float (&acc)[8][2][2], float* __restrict__ buf1, const Luma& lc) {
+  const float c0 = compress_one_literal(plane, i, 2 * lane, lc);
   // output column 127 does not exist: lane 63 evaluates column 126 again (in
   // bounds) and discards it
-  float c1 = compress_one_literal(plane, i, lane == 63 ? 126 : 2 * lane + 1);
+  float c1 = compress_one_literal(plane, i, lane == 63 ? 126 : 2 * lane + 1, lc);
   if (lane == 63) c1 = 0.0f;
   if (buf1) {
     buf1[i * 128 + 2 * lane] = c0;
@@ -291,7 +300,7 @@ __device__ __forceinline__ void literal_step(const T* __restrict__ plane, int la
 // stage-1 rows to W.t.  Rarely taken, kept out of line.
 template <typename T>
 __device__ __noinline__ void stream_image_literal(const T* __restrict__ plane, int lane, WaveLds& W,
-                                                  float* __restrict__ buf1) {
+                                                  float* __restrict__ buf1, Luma lc) {
   // two passes of eight frequencies each keep the accumulators small; the
   // compressed values are recomputed identically in the second pass
 #pragma unroll 1
@@ -303,14 +312,14 @@ __device__ __noinline__ void stream_image_literal(const T* __restrict__ plane, i
       for (int jj = 0; jj < 2; jj++) acc[k][jj][0] = acc[k][jj][1] = 0.0f;
 #pragma unroll 1
     for (int i = 0; i < 124; i += 4) {
-      literal_step<1>(plane, lane, i, k0, acc, buf1);
-      literal_step<0>(plane, lane, i + 1, k0, acc, buf1);
-      literal_step<0>(plane, lane, i + 2, k0, acc, buf1);
-      literal_step<1>(plane, lane, i + 3, k0, acc, buf1);
+      literal_step<1>(plane, lane, i, k0, acc, buf1, lc);
+      literal_step<0>(plane, lane, i + 1, k0, acc, buf1, lc);
+      literal_step<0>(plane, lane, i + 2, k0, acc, buf1, lc);
+      literal_step<1>(plane, lane, i + 3, k0, acc, buf1, lc);
     }
-    literal_step<1>(plane, lane, 124, k0, acc, buf1);
-    literal_step<0>(plane, lane, 125, k0, acc, buf1);
-    literal_step<0>(plane, lane, 126, k0, acc, buf1);
+    literal_step<1>(plane, lane, 124, k0, acc, buf1, lc);
+    literal_step<0>(plane, lane, 125, k0, acc, buf1, lc);
+    literal_step<0>(plane, lane, 126, k0, acc, buf1, lc);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
 #pragma unroll
@@ -352,22 +361,30 @@ __device__ __forceinline__ bool plane_needs_literal(const float* __restrict__ pl
   return wave_any(bad);
 }
 
+// u8 and colour inputs are finite by construction.
 __device__ __forceinline__ bool plane_needs_literal(const uint8_t* __restrict__, int, const float (&)[16][2][2]) {
+  return false;
+}
+__device__ __forceinline__ bool plane_needs_literal(const Rgb8* __restrict__, int, const float (&)[16][2][2]) {
+  return false;
+}
+__device__ __forceinline__ bool plane_needs_literal(const Rgba8* __restrict__, int, const float (&)[16][2][2]) {
   return false;
 }
 
 template <typename T>
 __device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t plane_stride, uint32_t img, int lane,
-                                         int wv, uint32_t mode, const HashOutputs& out, BlockLds& L) {
+                                         int wv, uint32_t mode, const HashOutputs& out, BlockLds& L,
+                                         const Luma& lc) {
   const T* plane = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(planes) + (size_t)img * plane_stride);
   float* buf1 = out.buf1 ? out.buf1 + (size_t)img * 128 * 128 : nullptr;
   WaveLds& W = L.w[wv];
   {
     float acc[16][2][2];
-    stream_image(plane, lane, acc, buf1);
+    stream_image(plane, lane, acc, buf1, lc);
     const bool literal = plane_needs_literal(plane, lane, acc);
     write_stage1(acc, W, lane);
-    if (literal) stream_image_literal(plane, lane, W, buf1);
+    if (literal) stream_image_literal(plane, lane, W, buf1, lc);
   }
   wave_lds_sync();
   if (buf1) {
@@ -387,7 +404,7 @@ __device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t 
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) pdq_hash_kernel(const T* __restrict__ planes, uint64_t plane_stride,
                                                         uint32_t n_images, const uint32_t* __restrict__ d_n,
-                                                        uint32_t mode, HashOutputs out) {
+                                                        uint32_t mode, HashOutputs out, Luma lc) {
   const uint32_t n = d_n ? min(*d_n, n_images) : n_images;
   if (blockIdx.x * kWavesPerBlock >= n) return;
   __shared__ BlockLds L;
@@ -402,7 +419,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) p
   __syncthreads();
   const uint32_t img = blockIdx.x * kWavesPerBlock + wv;
   if (img >= n) return;
-  hash_one(planes, plane_stride, img, lane, wv, mode, out, L);
+  hash_one(planes, plane_stride, img, lane, wv, mode, out, L, lc);
 }
 
 // Dihedrals of given PDQFs (the visit entry point): pdqf [n][256], thr_in [n].
@@ -431,37 +448,40 @@ hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, 
   return hipGetLastError();
 }
 
-template __global__ void pdq_hash_kernel<float>(const float* __restrict__, uint64_t, uint32_t, const uint32_t*,
-                                                uint32_t, HashOutputs);
-template __global__ void pdq_hash_kernel<uint8_t>(const uint8_t* __restrict__, uint64_t, uint32_t, const uint32_t*,
-                                                  uint32_t, HashOutputs);
+
+
+template <typename T>
+static void launch_kind(const void* planes, uint64_t plane_stride, uint32_t n_max, const uint32_t* d_n, uint32_t mode,
+                        const HashOutputs& out, const Luma& lc, hipStream_t stream) {
+  const dim3 grid((n_max + kWavesPerBlock - 1) / kWavesPerBlock);
+  hipLaunchKernelGGL(pdq_hash_kernel<T>, grid, dim3(64 * kWavesPerBlock), 0, stream, static_cast<const T*>(planes),
+                     plane_stride, n_max, d_n, mode, out, lc);
+}
+
+static hipError_t launch_any(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                             const uint32_t* d_n, uint32_t mode, const HashOutputs& out, const float* luma,
+                             hipStream_t stream) {
+  if (n_max == 0) return hipSuccess;
+  Luma lc{0.299f, 0.587f, 0.114f};
+  if (luma) lc = Luma{luma[0], luma[1], luma[2]};
+  switch (input_kind) {
+    case kInputF32: launch_kind<float>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputU8: launch_kind<uint8_t>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputRgb8: launch_kind<Rgb8>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputRgba8: launch_kind<Rgba8>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
-                       const HashOutputs& out, hipStream_t stream) {
-  if (n_images == 0) return hipSuccess;
-  const dim3 grid((n_images + kWavesPerBlock - 1) / kWavesPerBlock);
-  const dim3 block(64 * kWavesPerBlock);
-  if (input_kind == kInputF32)
-    hipLaunchKernelGGL(pdq_hash_kernel<float>, grid, block, 0, stream, static_cast<const float*>(planes),
-                       plane_stride, n_images, nullptr, mode, out);
-  else
-    hipLaunchKernelGGL(pdq_hash_kernel<uint8_t>, grid, block, 0, stream, static_cast<const uint8_t*>(planes),
-                       plane_stride, n_images, nullptr, mode, out);
-  return hipGetLastError();
+                       const HashOutputs& out, hipStream_t stream, const float* luma) {
+  return launch_any(planes, input_kind, plane_stride, n_images, nullptr, mode, out, luma, stream);
 }
 
 hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
                                     const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream) {
-  if (n_max == 0) return hipSuccess;
-  const dim3 g((n_max + kWavesPerBlock - 1) / kWavesPerBlock);
-  const dim3 block(64 * kWavesPerBlock);
-  if (input_kind == kInputF32)
-    hipLaunchKernelGGL(pdq_hash_kernel<float>, g, block, 0, stream, static_cast<const float*>(planes), plane_stride,
-                       n_max, d_n, mode, out);
-  else
-    hipLaunchKernelGGL(pdq_hash_kernel<uint8_t>, g, block, 0, stream, static_cast<const uint8_t*>(planes),
-                       plane_stride, n_max, d_n, mode, out);
-  return hipGetLastError();
+  return launch_any(planes, input_kind, plane_stride, n_max, d_n, mode, out, nullptr, stream);
 }
 
 }  // namespace ypdq

@@ -84,6 +84,89 @@ __device__ __forceinline__ void to_f32(const RowU8& in, Row8& o) {
   o.v[7] = (float)(in.hi >> 24);
 }
 
+// Colour inputs, converted to luma in the row loader: RGB8 by the Kernel
+// trait's default cvt_rgb8_to_luma8f (mod.rs:303-322: R*r + G*g + B*b, left
+// to right, no fused multiply-add), RGBA8 by the AVX2 kernel's
+// cvt_rgba8_to_luma8f (x86.rs:686-753: r*R, then fma(g, G, .), then
+// fma(b, B, .)).  Coefficients (R, G, B), BT.601 by default
+// (constants.rs:23).
+struct Luma {
+  float r, g, b;
+};
+struct Rgb8 {
+  uint8_t c[3];
+};
+struct Rgba8 {
+  uint8_t c[4];
+};
+
+__device__ __forceinline__ float luma_rgb(uint32_t r, uint32_t g, uint32_t b, const Luma& lc) {
+  const float pr = lc.r * (float)r;
+  const float pg = lc.g * (float)g;
+  const float pb = lc.b * (float)b;
+  return (pr + pg) + pb;
+}
+__device__ __forceinline__ float luma_rgba(uint32_t r, uint32_t g, uint32_t b, const Luma& lc) {
+  float v = (float)r * lc.r;
+  v = __builtin_fmaf((float)g, lc.g, v);
+  return __builtin_fmaf((float)b, lc.b, v);
+}
+
+struct RowRgb {  // 8 pixels x 3 bytes
+  uint32_t w[6];
+};
+struct RowRgba {  // 8 pixels x 4 bytes
+  uint32_t w[8];
+};
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load_raw(const Rgb8* __restrict__ plane, int r, int lane, RowRgb& o) {
+  // 1536 bytes per row, 24 per lane: three 8-byte loads
+  const u32x2* p = reinterpret_cast<const u32x2*>(reinterpret_cast<const uint8_t*>(plane) + (size_t)r * 1536 + 24 * lane);
+  const u32x2 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1),
+              c = __builtin_nontemporal_load(p + 2);
+  o.w[0] = a.x; o.w[1] = a.y; o.w[2] = b.x; o.w[3] = b.y; o.w[4] = c.x; o.w[5] = c.y;
+}
+
+__device__ __forceinline__ void load_raw(const Rgba8* __restrict__ plane, int r, int lane, RowRgba& o) {
+  const u32x4* p = reinterpret_cast<const u32x4*>(reinterpret_cast<const uint8_t*>(plane) + (size_t)r * 2048 + 32 * lane);
+  const u32x4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+  o.w[0] = a.x; o.w[1] = a.y; o.w[2] = a.z; o.w[3] = a.w;
+  o.w[4] = b.x; o.w[5] = b.y; o.w[6] = b.z; o.w[7] = b.w;
+}
+
+__device__ __forceinline__ void to_f32(const Row8& in, Row8& o, const Luma&) { o = in; }
+__device__ __forceinline__ void to_f32(const RowU8& in, Row8& o, const Luma&) { to_f32(in, o); }
+
+__device__ __forceinline__ void to_f32(const RowRgb& in, Row8& o, const Luma& lc) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    uint32_t ch[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int byte = 3 * k + c;
+      ch[c] = (in.w[byte >> 2] >> (8 * (byte & 3))) & 0xffu;
+    }
+    o.v[k] = luma_rgb(ch[0], ch[1], ch[2], lc);
+  }
+}
+
+__device__ __forceinline__ void to_f32(const RowRgba& in, Row8& o, const Luma& lc) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t w = in.w[k];
+    o.v[k] = luma_rgba(w & 0xffu, (w >> 8) & 0xffu, (w >> 16) & 0xffu, lc);
+  }
+}
+
+// One input element as f32 (the literal compression path).
+__device__ __forceinline__ float px_f32(float v, const Luma&) { return v; }
+__device__ __forceinline__ float px_f32(uint8_t v, const Luma&) { return (float)v; }
+__device__ __forceinline__ float px_f32(const Rgb8& v, const Luma& lc) { return luma_rgb(v.c[0], v.c[1], v.c[2], lc); }
+__device__ __forceinline__ float px_f32(const Rgba8& v, const Luma& lc) {
+  return luma_rgba(v.c[0], v.c[1], v.c[2], lc);
+}
+
 template <typename T>
 struct RawRow;
 template <>
@@ -93,6 +176,14 @@ struct RawRow<float> {
 template <>
 struct RawRow<uint8_t> {
   using type = RowU8;
+};
+template <>
+struct RawRow<Rgb8> {
+  using type = RowRgb;
+};
+template <>
+struct RawRow<Rgba8> {
+  using type = RowRgba;
 };
 
 // ---------------------------------------------------------------------------
@@ -155,7 +246,8 @@ __device__ __forceinline__ void chains_finish(const Chains& c, float& c0, float&
 // the one-element rewind at (126,126,di=6), reads past a row's end), used when
 // the plane holds a non-finite value.  Reads the plane from memory directly.
 template <typename T>
-__device__ __forceinline__ float compress_one_literal(const T* __restrict__ plane, int outi, int outj) {
+__device__ __forceinline__ float compress_one_literal(const T* __restrict__ plane, int outi, int outj,
+                                                      const Luma& lc = Luma{0.0f, 0.0f, 0.0f}) {
   const int in_i = 4 * (outi + 1) - 3;
   const int in_j = 4 * (outj + 1) - 3;
   float lane[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -165,11 +257,11 @@ __device__ __forceinline__ float compress_one_literal(const T* __restrict__ plan
     float x[8];
     if (di == 6 && outi == 126 && outj == 126) {
 #pragma unroll
-      for (int k = 0; k < 7; k++) x[k] = (float)plane[off + k];
-      x[7] = (float)plane[off + 6];
+      for (int k = 0; k < 7; k++) x[k] = px_f32(plane[off + k], lc);
+      x[7] = px_f32(plane[off + 6], lc);
     } else {
 #pragma unroll
-      for (int k = 0; k < 8; k++) x[k] = (float)plane[off + k];
+      for (int k = 0; k < 8; k++) x[k] = px_f32(plane[off + k], lc);
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) lane[k] = __builtin_fmaf(x[k], tw(di, k), lane[k]);

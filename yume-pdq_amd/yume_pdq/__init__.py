@@ -13,7 +13,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("LIBYUME_PDQ_PATH", os.path.join(PKG_DIR, "libyume_pdq.so"))
 
-INPUT_F32, INPUT_U8 = 0, 1
+INPUT_F32, INPUT_U8, INPUT_RGB8, INPUT_RGBA8 = 0, 1, 2, 3
 DIHEDRAL_EXACT, DIHEDRAL_AVX2 = 0, 1
 MATCH_APPEND = 1
 MATCH_VALU = 2
@@ -90,6 +90,8 @@ def lib():
         L.yume_pdq_amd_device_info.argtypes = [i, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
         L.yume_pdq_amd_hash_batch.restype = i
         L.yume_pdq_amd_hash_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_hash_batch_luma.restype = i
+        L.yume_pdq_amd_hash_batch_luma.argtypes = [vp, i, u64, u32, u32, C.POINTER(C.c_float), vp, vp, vp, vp, vp]
         L.yume_pdq_amd_dihedrals_batch.restype = i
         L.yume_pdq_amd_dihedrals_batch.argtypes = [vp, vp, u32, u32, vp, vp, vp, vp, vp]
         L.yume_pdq_amd_match.restype = i
@@ -218,29 +220,44 @@ def _stream_ptr(stream):
     return C.c_void_p(stream.cuda_stream)
 
 
-def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None):
-    """Hash n 512x512 planes resident on the GPU (torch f32 or u8, shape (n,512,512)).
+def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None, luma=None):
+    """Hash n 512x512 planes resident on the GPU: torch f32 or u8 luma
+    (n, 512, 512), or u8 RGB8 / RGBA8 pixels (n, 512, 512, 3 or 4) converted to
+    luma while read, with (R, G, B) coefficients `luma` (default BT.601).
 
     Returns (hashes (n,8,32) u8, quality (n,8) f32, threshold (n,8) f32[, pdqf (n,16,16)])."""
     torch = _torch()
-    if planes.dim() == 2:
+    colour = planes.dim() >= 3 and planes.shape[-1] in (3, 4) and planes.shape[-3:-1] == (512, 512)
+    if colour and planes.dim() == 3:
         planes = planes.unsqueeze(0)
-    if planes.shape[-2:] != (512, 512) or not planes.is_cuda:
-        raise ValueError("planes must be a CUDA tensor of shape (n, 512, 512)")
-    if planes.dtype == torch.float32:
-        kind, elem = INPUT_F32, 4
+    if not colour and planes.dim() == 2:
+        planes = planes.unsqueeze(0)
+    if not planes.is_cuda or (planes.shape[1:3] != (512, 512)):
+        raise ValueError("planes must be a CUDA tensor of shape (n, 512, 512[, 3|4])")
+    if colour:
+        if planes.dtype != torch.uint8:
+            raise ValueError("colour planes must be uint8")
+        kind, elem = (INPUT_RGB8, 3) if planes.shape[-1] == 3 else (INPUT_RGBA8, 4)
+        planes = planes.contiguous()
+    elif planes.dtype == torch.float32:
+        kind, elem = INPUT_F32, 1
     elif planes.dtype == torch.uint8:
         kind, elem = INPUT_U8, 1
     else:
         raise ValueError("planes must be float32 or uint8")
-    if planes.stride(-1) != 1 or planes.stride(-2) != 512:
+    if planes.dtype == torch.float32:
+        elem = 4
+    if not colour and (planes.stride(-1) != 1 or planes.stride(-2) != 512):
         planes = planes.contiguous()
     n = planes.shape[0]
+    per = 512 * 512 * (planes.shape[-1] if colour else 1)  # elements per plane
     # a size-1 batch dimension may carry any stride (e.g. 0 from x[None])
-    stride = planes.stride(0) if n > 1 else 512 * 512
-    if stride < 512 * 512:
+    stride = planes.stride(0) if n > 1 else per
+    if stride < per:
         planes = planes.contiguous()
-        stride = 512 * 512
+        stride = per
+    if colour:
+        elem = 1  # stride counted in bytes of u8 elements
     dev = planes.device
     if out is None:
         hashes = torch.empty((n, 8, 32), dtype=torch.uint8, device=dev)
@@ -249,11 +266,19 @@ def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=Non
         pdqf = torch.empty((n, 16, 16), dtype=torch.float32, device=dev) if want_pdqf else None
     else:
         hashes, quality, threshold, pdqf = out
-    _check(lib().yume_pdq_amd_hash_batch(
-        C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode,
-        C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
-        C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
-        _stream_ptr(stream)), "hash_batch")
+    if luma is None:
+        _check(lib().yume_pdq_amd_hash_batch(
+            C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode,
+            C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+            C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
+            _stream_ptr(stream)), "hash_batch")
+    else:
+        coeff = (C.c_float * 3)(*[float(x) for x in luma])
+        _check(lib().yume_pdq_amd_hash_batch_luma(
+            C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode, coeff,
+            C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+            C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
+            _stream_ptr(stream)), "hash_batch_luma")
     if want_pdqf:
         return hashes, quality, threshold, pdqf
     return hashes, quality, threshold
