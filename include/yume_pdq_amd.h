@@ -104,6 +104,46 @@ int yume_pdq_amd_hash_batch_luma(const void *d_planes, int input_kind, uint64_t 
                                  uint8_t *d_hashes, float *d_quality, float *d_threshold, float *d_pdqf,
                                  void *stream);
 
+/* DCT arithmetic of yume_pdq_amd_hash_batch_ex.  EXACT is the reference's
+ * FP32 operation order (bit-identical hashes).  The others run both DCT
+ * stages on matrix cores with operands rounded to bf16 / f16 and f32
+ * accumulation; the X3 forms split each operand into hi + lo and sum three
+ * products.  Their hashes may differ from the reference's in bits whose
+ * coefficient lies near the median; quality and threshold stay FP32
+ * computations over the approximate coefficients.  Planes holding NaN/Inf
+ * always take the exact arithmetic. */
+enum {
+  YUME_PDQ_AMD_DCT_EXACT = 0,
+  YUME_PDQ_AMD_DCT_BF16 = 1,
+  YUME_PDQ_AMD_DCT_F16 = 2,
+  YUME_PDQ_AMD_DCT_BF16X3 = 3,
+  YUME_PDQ_AMD_DCT_F16X3 = 4
+};
+/* Kernel layout: one wavefront per image, or 4 / 8 wavefronts sharing an
+ * image (lower latency per image, fills the GPU with fewer images).  AUTO
+ * picks by batch size.  The exact results do not depend on the layout. */
+enum {
+  YUME_PDQ_AMD_LAYOUT_AUTO = 0,
+  YUME_PDQ_AMD_LAYOUT_WAVE = 1,
+  YUME_PDQ_AMD_LAYOUT_BLOCK4 = 2,
+  YUME_PDQ_AMD_LAYOUT_BLOCK8 = 3
+};
+typedef struct YumePdqAmdHashOptions {
+  uint32_t dihedral_mode;   /* YUME_PDQ_AMD_DIHEDRAL_* */
+  uint32_t dct_precision;   /* YUME_PDQ_AMD_DCT_* (not EXACT: a BLOCK layout) */
+  uint32_t layout;          /* YUME_PDQ_AMD_LAYOUT_* */
+  const float *luma_coeff;  /* (R, G, B) for colour inputs, NULL = BT.601 */
+} YumePdqAmdHashOptions;
+
+/* yume_pdq_amd_hash_batch with options (NULL = the defaults of
+ * yume_pdq_amd_hash_batch).  d_buf1: n_images x 128 x 128 floats of each
+ * compressed plane (jarosz_compress output with row and column 127 zero), or
+ * NULL. */
+int yume_pdq_amd_hash_batch_ex(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
+                               uint32_t n_images, const YumePdqAmdHashOptions *opts, uint8_t *d_hashes,
+                               float *d_quality, float *d_threshold, float *d_pdqf, float *d_buf1,
+                               void *stream);
+
 /*
  * Batched visit_dihedrals (src/lib.rs:358-420) from given PDQFs.
  * d_pdqf: n x 256; d_thr_in: n thresholds (as returned with each PDQF).

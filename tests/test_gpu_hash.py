@@ -37,10 +37,17 @@ def oracle_batch(oracle, planes_f32, mode):
 
 
 def check_batch(gpu, oracle, planes_f32, torch_planes, mode, label):
-    say(f"[hash] {label}: n={planes_f32.shape[0]} dtype={torch_planes.dtype} mode={mode}")
-    h, q, t, p = gpu.hash_batch(torch_planes, dihedral_mode=mode, want_pdqf=True)
-    h, q, t, p = (x.cpu().numpy() for x in (h, q, t, p))
+    """Both kernel layouts (a wave per image; eight waves per image, the
+    automatic choice for small batches) against the oracle."""
     H, Q, T, P = oracle_batch(oracle, planes_f32, mode)
+    for layout in (gpu.LAYOUT_WAVE, gpu.LAYOUT_BLOCK8):
+        say(f"[hash] {label}: n={planes_f32.shape[0]} dtype={torch_planes.dtype} mode={mode} layout={layout}")
+        h, q, t, p = gpu.hash_batch(torch_planes, dihedral_mode=mode, want_pdqf=True, layout=layout)
+        h, q, t, p = (x.cpu().numpy() for x in (h, q, t, p))
+        check_against(planes_f32, h, q, t, p, H, Q, T, P, f"{label} layout={layout}")
+
+
+def check_against(planes_f32, h, q, t, p, H, Q, T, P, label):
     for i in range(planes_f32.shape[0]):
         assert np.array_equal(h[i], H[i]), f"{label} image {i}: hash mismatch " \
             f"{[int(np.unpackbits(h[i][d] ^ H[i][d]).sum()) for d in range(8)]}"

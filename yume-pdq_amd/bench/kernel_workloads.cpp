@@ -7,7 +7,10 @@
 //   match_x1024     exhaustive match of 128 images' 1024 dihedral hashes
 //   match_plane_x8, match_plane_x1024  the same over the database's prefix plane
 //   build_plane     building the prefix plane (not in the default set)
-//   hash_u8         fused hash of --batch u8 planes
+//   hash_u8         fused hash of --batch u8 planes (automatic layout)
+//   hash_u8_wave, hash_u8_block4, hash_u8_block8   the same in one layout
+//   hash_u8_bf16, hash_u8_bf16x3, hash_u8_f16, hash_u8_f16x3   the same with
+//                   the matrix-core DCT (eight waves per image)
 // The database holds 50,000 planted near-duplicates (U{0..40} distinct bit
 // flips) drawn from a pool of 8192 hashes whose first 1024 are the queries.
 // Each case: W untimed launches, then K launches between two HIP events on
@@ -194,6 +197,25 @@ int main(int argc, char** argv) {
                                                   YUME_PDQ_AMD_DIHEDRAL_EXACT, d_hashes, d_q, d_t, nullptr, s),
                           "hash_batch");
                  }});
+  struct Variant {
+    const char* name;
+    uint32_t dct, layout;
+  };
+  const Variant variants[] = {{"hash_u8_wave", YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_WAVE},
+                              {"hash_u8_block4", YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_BLOCK4},
+                              {"hash_u8_block8", YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_BLOCK8},
+                              {"hash_u8_bf16", YUME_PDQ_AMD_DCT_BF16, YUME_PDQ_AMD_LAYOUT_BLOCK8},
+                              {"hash_u8_bf16x3", YUME_PDQ_AMD_DCT_BF16X3, YUME_PDQ_AMD_LAYOUT_BLOCK8},
+                              {"hash_u8_f16", YUME_PDQ_AMD_DCT_F16, YUME_PDQ_AMD_LAYOUT_BLOCK8},
+                              {"hash_u8_f16x3", YUME_PDQ_AMD_DCT_F16X3, YUME_PDQ_AMD_LAYOUT_BLOCK8}};
+  for (const Variant& v : variants) {
+    all.push_back({v.name, [&, v] {
+                     YumePdqAmdHashOptions o = {YUME_PDQ_AMD_DIHEDRAL_EXACT, v.dct, v.layout, nullptr};
+                     api_ok(yume_pdq_amd_hash_batch_ex(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, batch, &o,
+                                                       d_hashes, d_q, d_t, nullptr, nullptr, s),
+                            "hash_batch_ex");
+                   }});
+  }
 
   hipEvent_t e0, e1;
   hip_ok(hipEventCreate(&e0), "event");

@@ -262,6 +262,42 @@ int yume_pdq_amd_hash_batch(const void* d_planes, int input_kind, uint64_t plane
                                     static_cast<hipStream_t>(stream)));
 }
 
+int yume_pdq_amd_hash_batch_ex(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
+                               const YumePdqAmdHashOptions* opts, uint8_t* d_hashes, float* d_quality,
+                               float* d_threshold, float* d_pdqf, float* d_buf1, void* stream) {
+  const YumePdqAmdHashOptions def = {YUME_PDQ_AMD_DIHEDRAL_EXACT, YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_AUTO,
+                                     nullptr};
+  const YumePdqAmdHashOptions& o = opts ? *opts : def;
+  if (n_images == 0) return YUME_PDQ_AMD_OK;
+  if (!d_planes || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
+  if (plane_bytes == 0) return YUME_PDQ_AMD_EINVAL;
+  if (o.dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
+  if (o.dct_precision > YUME_PDQ_AMD_DCT_F16X3 || o.layout > YUME_PDQ_AMD_LAYOUT_BLOCK8) return YUME_PDQ_AMD_EINVAL;
+  if (o.layout == YUME_PDQ_AMD_LAYOUT_WAVE && o.dct_precision != YUME_PDQ_AMD_DCT_EXACT) return YUME_PDQ_AMD_EINVAL;
+  if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
+    return YUME_PDQ_AMD_EINVAL;
+  HashOutputs out;
+  out.hashes = d_hashes;
+  out.quality = d_quality;
+  out.threshold = d_threshold;
+  out.pdqf = d_pdqf;
+  out.buf1 = d_buf1;
+  out.tmp = nullptr;
+  out.dihedral_pdqf = nullptr;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  int layout = (int)o.layout;
+  if (layout == YUME_PDQ_AMD_LAYOUT_AUTO)
+    layout = ypdq::hash_layout_for(n_images, (int)o.dct_precision) == 1 ? YUME_PDQ_AMD_LAYOUT_WAVE
+                                                                       : YUME_PDQ_AMD_LAYOUT_BLOCK8;
+  if (layout == YUME_PDQ_AMD_LAYOUT_WAVE)
+    return from_hip(ypdq::launch_hash_wave(d_planes, input_kind, plane_stride_bytes, n_images, nullptr,
+                                           o.dihedral_mode, out, o.luma_coeff, s));
+  const int waves = layout == YUME_PDQ_AMD_LAYOUT_BLOCK8 ? 8 : 4;
+  return from_hip(ypdq::launch_hash_block(d_planes, input_kind, plane_stride_bytes, n_images, nullptr,
+                                          o.dihedral_mode, (int)o.dct_precision, waves, out, o.luma_coeff, s));
+}
+
 int yume_pdq_amd_dihedrals_batch(const float* d_pdqf, const float* d_thr_in, uint32_t n, uint32_t dihedral_mode,
                                  uint8_t* d_hashes, float* d_quality, float* d_threshold, float* d_dihedral_pdqf,
                                  void* stream) {

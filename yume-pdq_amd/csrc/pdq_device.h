@@ -51,12 +51,30 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ bool wave_any(bool b) { return __any(b ? 1 : 0) != 0; }
 __device__ __forceinline__ bool wave_all(bool b) { return __all(b ? 1 : 0) != 0; }
 
-// luma: (R, G, B) coefficients of the colour inputs, null for BT.601.
+// Fused hash of n_images planes, exact arithmetic, the layout chosen by
+// hash_layout_for.  luma: (R, G, B) coefficients of the colour inputs, null
+// for BT.601.
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
                        const HashOutputs& out, hipStream_t stream, const float* luma = nullptr);
 // Grid for n_max images, image count read from d_n (capped at n_max).
 hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
                                     const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream);
+// DCT arithmetic of the block-per-image hash kernel: the reference's exact
+// FP32 chains, or matrix cores with bf16 / f16 operands (x3: hi + lo split
+// operands, three products) and f32 accumulation.
+enum DctPrecision { kDctExact = 0, kDctBf16 = 1, kDctF16 = 2, kDctBf16x3 = 3, kDctF16x3 = 4 };
+// Wave-per-image form (d_n: device image count capped at n_max, or null).
+hipError_t launch_hash_wave(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                            const uint32_t* d_n, uint32_t mode, const HashOutputs& out, const float* luma,
+                            hipStream_t stream);
+// Block-per-image form: `waves` (4 or 8) wavefronts per image; d_n as in
+// launch_hash_device_count (or null); luma null for BT.601.
+hipError_t launch_hash_block(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                             const uint32_t* d_n, uint32_t mode, int dct, int waves, const HashOutputs& out,
+                             const float* luma, hipStream_t stream);
+// Layout the automatic choice uses for n images: 1 = wave per image,
+// 8 = block of eight waves per image.
+int hash_layout_for(uint32_t n_images, int dct);
 hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, uint32_t mode,
                             const HashOutputs& out, hipStream_t stream);
 // Shape of a matcher launch: rows per lane per iteration, threads per block,

@@ -24,137 +24,9 @@
 #include <stdint.h>
 
 #include "pdq_hash_common.h"
+#include "pdq_hash_tail.h"
 
 namespace ypdq {
-// ---------------------------------------------------------------------------
-// The tail: stage 2, dihedrals, gradients, quantisation, packing.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void stage2_to_pdqf(const BlockLds& L, WaveLds& W, int lane, const HashOutputs& out,
-                                               uint32_t img) {
-  if (out.tmp) {
-    // the reference leaves the k = 15 stage-1 row in its tmp buffer
-    float* dst = out.tmp + (size_t)img * 128;
-    dst[2 * lane] = W.t[15 * 128 + 2 * lane];
-    dst[2 * lane + 1] = W.t[15 * 128 + 2 * lane + 1];
-  }
-  // Stage 2 (x86.rs:227-272): lane (k = lane & 15, jq = lane >> 4) computes
-  // out[k][4jq + r], r = 0..3, one output at a time: 8 AVX2 lanes x 2
-  // alternating accumulators over 16 chunks, then the horizontal tree.
-  const int k = lane & 15, jq = lane >> 4;
-#pragma unroll 1
-  for (int r = 0; r < 4; r++) {
-    const int j = 4 * jq + r;
-    float h[2][8];
-#pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-      for (int l = 0; l < 8; l++) h[p][l] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-      const f32x4* tp = reinterpret_cast<const f32x4*>(&W.t[k * 128 + 8 * c]);
-      const f32x4* dp = reinterpret_cast<const f32x4*>(&L.d2[j * 128 + 8 * c]);
-      const f32x4 ta = tp[0], tb = tp[1], da = dp[0], db = dp[1];
-      const float tv[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
-      const float dv[8] = {da.x, da.y, da.z, da.w, db.x, db.y, db.z, db.w};
-      const int p = alt(c);
-#pragma unroll
-      for (int l = 0; l < 8; l++) h[p][l] = __builtin_fmaf(tv[l], dv[l], h[p][l]);
-    }
-    float v[8];
-#pragma unroll
-    for (int l = 0; l < 8; l++) v[l] = h[0][l] + h[1][l];
-    W.p[k * 16 + j] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-  }
-  wave_lds_sync();
-}
-
-// From the NORMAL PDQF in W.p: the eight dihedral PDQFs, their qualities,
-// thresholds and hashes.  thr_normal < 0 (null) means "use NORMAL's own
-// quantisation threshold" for ROTATED_90_FLOPPED; the visit entry point passes
-// the caller's threshold instead (src/lib.rs:403-404).
-__device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mode, const HashOutputs& out, uint32_t img,
-                              const float* thr_normal) {
-  // The eight dihedral PDQFs into W.t (d-major, 256 each):
-  //   0 NORMAL P          1 FLOPPED  sc.P       2 ROTATED_180 sr.sc.P   3 FLIPPED sr.P
-  //   4 ROT90_FLOPPED P^T 5 ROT270   sr.P^T     6 FLOPPED_ROT270 sr.sc.P^T  7 ROT90 sc.P^T
-  // sr negates even rows (pdqf_negate_alt_rows::<true>, x86.rs:285-326); sc
-  // negates even columns (pdqf_negate_alt_cols::<false>) -- all 16 in exact
-  // mode, columns 0..7 only in the AVX2-compatible mode (x86.rs:330-376 loads
-  // 8 floats per row).  Negation is exact, so the sequence of in-place flips
-  // in visit_dihedrals reduces to these closed forms.
-  {
-    const int ncols = mode == kDihedralAvx2 ? 8 : 16;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int idx = lane + 64 * e;  // element (i, j) of the output orientation
-      const int i = idx >> 4, j = idx & 15;
-      const float a = W.p[idx];
-      const float at = W.p[j * 16 + i];
-      const bool nr = (i & 1) == 0;
-      const bool nc = (j & 1) == 0 && j < ncols;
-      W.t[0 * 256 + idx] = a;
-      W.t[1 * 256 + idx] = nc ? a * -1.0f : a;
-      W.t[2 * 256 + idx] = (nr != nc) ? a * -1.0f : a;
-      W.t[3 * 256 + idx] = nr ? a * -1.0f : a;
-      W.t[4 * 256 + idx] = at;
-      W.t[5 * 256 + idx] = nr ? at * -1.0f : at;
-      W.t[6 * 256 + idx] = (nr != nc) ? at * -1.0f : at;
-      W.t[7 * 256 + idx] = nc ? at * -1.0f : at;
-    }
-  }
-  wave_lds_sync();
-
-  // Gradient sums (x86.rs:840-865), one sequential accumulation per
-  // dihedral, lane d owns dihedral d.
-  if (lane < 8) W.q[lane] = adjust_quality(sum_of_gradients16(&W.t[lane * 256]));
-
-  // Median search (x86.rs:440-621) for dihedrals 0..3: group s = lane >> 3
-  // (lanes 32..63 repeat groups 0..3).
-  {
-    const int s = (lane >> 3) & 3;
-    float thr, bthr;
-    median_search8(&W.t[s * 256], lane, thr, bthr);
-    if (lane < 32 && (lane & 7) == 0) {
-      W.th[s] = thr;
-      W.bt[s] = bthr;
-    }
-  }
-  wave_lds_sync();
-  if (lane < 4) {
-    // threshold_2d_f32 with the thresholds visit_dihedrals passes (lib.rs:403-416)
-    const float t = (lane == 0 && thr_normal) ? *thr_normal : W.th[lane];
-    W.th[4 + lane] = t;
-    W.bt[4 + lane] = t;
-  }
-  wave_lds_sync();
-
-  // Bits: lane (d = lane >> 3, w = lane & 7) builds bytes 4w..4w+3 of hash d;
-  // byte b holds elements 8(31-b) .. 8(31-b)+7 at bits 0..7 (x86.rs:497-503).
-  {
-    const int d = lane >> 3, w = lane & 7;
-    const float* P = &W.t[d * 256];
-    const float bt = W.bt[d];
-    uint32_t word = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) word |= hash_byte(P, 4 * w + q, bt) << (8 * q);
-    reinterpret_cast<uint32_t*>(out.hashes + (size_t)img * 256 + d * 32)[w] = word;
-  }
-  if (lane < 8) {
-    out.quality[(size_t)img * 8 + lane] = W.q[lane];
-    out.threshold[(size_t)img * 8 + lane] = W.th[lane];
-  }
-  if (out.pdqf) {
-#pragma unroll
-    for (int e = 0; e < 4; e++) out.pdqf[(size_t)img * 256 + lane + 64 * e] = W.p[lane + 64 * e];
-  }
-  if (out.dihedral_pdqf) {
-    float* dst = out.dihedral_pdqf + (size_t)img * 8 * 256;
-#pragma unroll
-    for (int e = 0; e < 32; e++) dst[lane + 64 * e] = W.t[lane + 64 * e];
-  }
-  wave_lds_sync();
-}
-
 // ---------------------------------------------------------------------------
 // Compress + stage 1 over one plane.
 // ---------------------------------------------------------------------------
@@ -393,7 +265,7 @@ __device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t 
     buf1[(2 * lane) * 128 + 127] = 0.0f;
     buf1[(2 * lane + 1) * 128 + 127] = 0.0f;
   }
-  stage2_to_pdqf(L, W, lane, out, img);
+  stage2_to_pdqf(L.d2, W, lane, out, img);
   dihedral_tail(W, lane, mode, out, img, nullptr);
 }
 
@@ -474,14 +346,10 @@ static hipError_t launch_any(const void* planes, int input_kind, uint64_t plane_
   return hipGetLastError();
 }
 
-hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
-                       const HashOutputs& out, hipStream_t stream, const float* luma) {
-  return launch_any(planes, input_kind, plane_stride, n_images, nullptr, mode, out, luma, stream);
-}
-
-hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
-                                    const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream) {
-  return launch_any(planes, input_kind, plane_stride, n_max, d_n, mode, out, nullptr, stream);
+hipError_t launch_hash_wave(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                            const uint32_t* d_n, uint32_t mode, const HashOutputs& out, const float* luma,
+                            hipStream_t stream) {
+  return launch_any(planes, input_kind, plane_stride, n_max, d_n, mode, out, luma, stream);
 }
 
 }  // namespace ypdq

@@ -18,6 +18,9 @@ DIHEDRAL_EXACT, DIHEDRAL_AVX2 = 0, 1
 MATCH_APPEND = 1
 MATCH_VALU = 2
 MATCH_MFMA = 4
+# DCT arithmetic and kernel layout of hash_batch (YUME_PDQ_AMD_DCT_*, _LAYOUT_*)
+DCT_EXACT, DCT_BF16, DCT_F16, DCT_BF16X3, DCT_F16X3 = 0, 1, 2, 3, 4
+LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_BLOCK4, LAYOUT_BLOCK8 = 0, 1, 2, 3
 
 # Orientation order of the 8 hashes per image (YUME_PDQ_AMD_D_*).
 DIHEDRALS = ["NORMAL", "FLOPPED", "ROTATED_180", "FLIPPED", "ROTATED_90_FLOPPED",
@@ -65,6 +68,11 @@ class ServerStats(C.Structure):
                 ("images", C.c_uint64)]
 
 
+class HashOptions(C.Structure):
+    _fields_ = [("dihedral_mode", C.c_uint32), ("dct_precision", C.c_uint32), ("layout", C.c_uint32),
+                ("luma_coeff", C.POINTER(C.c_float))]
+
+
 class TuneRecord(C.Structure):
     _fields_ = [("n_queries", C.c_uint32), ("path", C.c_uint32), ("query_tiles", C.c_uint32), ("ms", C.c_float)]
 
@@ -90,6 +98,8 @@ def lib():
         L.yume_pdq_amd_device_info.argtypes = [i, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
         L.yume_pdq_amd_hash_batch.restype = i
         L.yume_pdq_amd_hash_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_hash_batch_ex.restype = i
+        L.yume_pdq_amd_hash_batch_ex.argtypes = [vp, i, u64, u32, C.POINTER(HashOptions), vp, vp, vp, vp, vp, vp]
         L.yume_pdq_amd_hash_batch_luma.restype = i
         L.yume_pdq_amd_hash_batch_luma.argtypes = [vp, i, u64, u32, u32, C.POINTER(C.c_float), vp, vp, vp, vp, vp]
         L.yume_pdq_amd_dihedrals_batch.restype = i
@@ -220,12 +230,19 @@ def _stream_ptr(stream):
     return C.c_void_p(stream.cuda_stream)
 
 
-def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None, luma=None):
+def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None, luma=None,
+               dct=DCT_EXACT, layout=LAYOUT_AUTO, want_buf1=False):
     """Hash n 512x512 planes resident on the GPU: torch f32 or u8 luma
     (n, 512, 512), or u8 RGB8 / RGBA8 pixels (n, 512, 512, 3 or 4) converted to
     luma while read, with (R, G, B) coefficients `luma` (default BT.601).
 
-    Returns (hashes (n,8,32) u8, quality (n,8) f32, threshold (n,8) f32[, pdqf (n,16,16)])."""
+    dct: DCT_EXACT (the reference's FP32 arithmetic, bit-identical) or a
+    matrix-core precision (DCT_BF16, DCT_F16, DCT_BF16X3, DCT_F16X3); layout:
+    LAYOUT_AUTO, LAYOUT_WAVE (one wavefront per image), LAYOUT_BLOCK4 or
+    LAYOUT_BLOCK8 (four or eight wavefronts per image).
+
+    Returns (hashes (n,8,32) u8, quality (n,8) f32, threshold (n,8) f32
+    [, pdqf (n,16,16)][, buf1 (n,128,128) compressed planes])."""
     torch = _torch()
     colour = planes.dim() >= 3 and planes.shape[-1] in (3, 4) and planes.shape[-3:-1] == (512, 512)
     if colour and planes.dim() == 3:
@@ -266,22 +283,34 @@ def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=Non
         pdqf = torch.empty((n, 16, 16), dtype=torch.float32, device=dev) if want_pdqf else None
     else:
         hashes, quality, threshold, pdqf = out
-    if luma is None:
-        _check(lib().yume_pdq_amd_hash_batch(
-            C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode,
-            C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
-            C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
-            _stream_ptr(stream)), "hash_batch")
+    buf1 = torch.empty((n, 128, 128), dtype=torch.float32, device=dev) if want_buf1 else None
+    pd = C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0)
+    if dct == DCT_EXACT and layout == LAYOUT_AUTO and not want_buf1:
+        if luma is None:
+            _check(lib().yume_pdq_amd_hash_batch(
+                C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode,
+                C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+                C.c_void_p(threshold.data_ptr()), pd, _stream_ptr(stream)), "hash_batch")
+        else:
+            coeff = (C.c_float * 3)(*[float(x) for x in luma])
+            _check(lib().yume_pdq_amd_hash_batch_luma(
+                C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode, coeff,
+                C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
+                C.c_void_p(threshold.data_ptr()), pd, _stream_ptr(stream)), "hash_batch_luma")
     else:
-        coeff = (C.c_float * 3)(*[float(x) for x in luma])
-        _check(lib().yume_pdq_amd_hash_batch_luma(
-            C.c_void_p(planes.data_ptr()), kind, stride * elem, n, dihedral_mode, coeff,
+        coeff = (C.c_float * 3)(*[float(x) for x in luma]) if luma is not None else None
+        opts = HashOptions(dihedral_mode, dct, layout, C.cast(coeff, C.POINTER(C.c_float)) if coeff else None)
+        _check(lib().yume_pdq_amd_hash_batch_ex(
+            C.c_void_p(planes.data_ptr()), kind, stride * elem, n, C.byref(opts),
             C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()),
-            C.c_void_p(threshold.data_ptr()), C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0),
-            _stream_ptr(stream)), "hash_batch_luma")
+            C.c_void_p(threshold.data_ptr()), pd,
+            C.c_void_p(buf1.data_ptr() if buf1 is not None else 0), _stream_ptr(stream)), "hash_batch_ex")
+    res = (hashes, quality, threshold)
     if want_pdqf:
-        return hashes, quality, threshold, pdqf
-    return hashes, quality, threshold
+        res = res + (pdqf,)
+    if want_buf1:
+        res = res + (buf1,)
+    return res
 
 
 def dihedrals_batch(pdqf, thr_in, dihedral_mode=DIHEDRAL_EXACT, stream=None):
