@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--verify", action="store_true", help="check B=1 hits against numpy (slow)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sections")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batching section")
+    ap.add_argument("--no-hash-precisions", action="store_true",
+                    help="skip the batched-hashing section (4096 images, every DCT precision)")
+    ap.add_argument("--hash-images", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-offered-10m", type=float, default=60_000.0,
                     help="offered images/s of the 10M-row serving run")
@@ -261,8 +264,10 @@ def main():
     # Hash algorithmic bytes: the u8 plane (262,144 B) + 8 x (32 + 4 + 4) B out.
     hash_bytes = (262144.0 + 320.0) * B
     hash_gbs = hash_bytes / (t_hash_head * 1e-3) / 1e9
+    layout_head = "pdq_hash_kernel<u8> (one wave per image)" if B >= 1024 else \
+        "pdq_hash_block_kernel<u8> (eight waves per image)"
     roofline_hash = {
-        "kernel": f"pdq_hash_kernel<u8>: {B} images per launch",
+        "kernel": f"{layout_head}: {B} images per launch",
         "bound": "hbm", "achieved": round(hash_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(hash_gbs / HBM_PEAK_GBS, 4),
         "traffic": counters.get(f"hash_u8_b{B}", {}).get("traffic_bytes"),
@@ -294,6 +299,10 @@ def main():
     uneven = None
     if not args.no_uneven:
         uneven = uneven_section(args, yume_pdq, torch, dev, synthetic, planes_u8[:256], pool_hashes, thr)
+
+    precisions = None
+    if not args.no_hash_precisions:
+        precisions = hash_precision_section(args, yume_pdq, torch, dev, synthetic, planes_u8)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -340,6 +349,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "uneven_batching": uneven,
+        "batched_hashing": precisions,
         "drop_in_ffi": {
             "us_per_hash": round(ffi_us, 2),
             "note": ("PCIe-inclusive per-image rate of the reference ABI: 1 MiB f32 host image "
@@ -521,6 +531,103 @@ def uneven_section(args, yume_pdq, torch, dev, synthetic, planes_u8, pool_hashes
     }
     log(f"[uneven]   {out}")
     return out
+
+
+def dct64_matrix(torch, dev):
+    """D[i][j] = sqrt(2/127) cos(pi (i+1)(2j+1) / 254), i < 16, j < 127, in FP64."""
+    i = torch.arange(1, 17, dtype=torch.float64, device=dev)[:, None]
+    j = torch.arange(127, dtype=torch.float64, device=dev)[None, :]
+    return torch.sqrt(torch.tensor(2.0 / 127.0, dtype=torch.float64, device=dev)) * \
+        torch.cos(torch.pi / 254.0 * i * (2.0 * j + 1.0))
+
+
+def hash_precision_section(args, yume_pdq, torch, dev, synthetic, planes_u8):
+    """Batched hashing of n synthetic images (u8 planes and the same images
+    as f32) in every kernel layout and DCT precision: kernel rate against the
+    HBM roofline, and for the matrix-core precisions the error of each PDQF
+    coefficient against the FP64 DCT of the same (exact) compressed plane --
+    computed on the GPU in FP64 as a verification oracle -- and the hash bits
+    that differ from the exact kernel's (= the reference's)."""
+    n = args.hash_images
+    log(f"[bench] batched hashing: {n} images, every layout and DCT precision")
+    have = planes_u8.shape[0]
+    if have < n:
+        extra = synthetic.images_u8_device(n - have, dev, start=have, seed=199)
+        u8 = torch.cat([planes_u8, extra]).contiguous()
+    else:
+        u8 = planes_u8[:n].contiguous()
+    f32 = u8.to(torch.float32)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    variants = [("exact", yume_pdq.DCT_EXACT, yume_pdq.LAYOUT_AUTO),
+                ("exact_wave", yume_pdq.DCT_EXACT, yume_pdq.LAYOUT_WAVE),
+                ("exact_block8", yume_pdq.DCT_EXACT, yume_pdq.LAYOUT_BLOCK8),
+                ("bf16", yume_pdq.DCT_BF16, yume_pdq.LAYOUT_BLOCK8),
+                ("f16", yume_pdq.DCT_F16, yume_pdq.LAYOUT_BLOCK8),
+                ("bf16x3", yume_pdq.DCT_BF16X3, yume_pdq.LAYOUT_BLOCK8),
+                ("f16x3", yume_pdq.DCT_F16X3, yume_pdq.LAYOUT_BLOCK8)]
+    outs = (torch.empty((n, 8, 32), dtype=torch.uint8, device=dev),
+            torch.empty((n, 8), dtype=torch.float32, device=dev),
+            torch.empty((n, 8), dtype=torch.float32, device=dev), None)
+    rates = {}
+    for kind, x, plane_bytes in (("u8", u8, 262144.0), ("f32", f32, 1048576.0)):
+        for name, dct, layout in variants:
+            def launch():
+                yume_pdq.hash_batch(x, stream=stream, out=outs, dct=dct, layout=layout)
+            for _ in range(2):
+                launch()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(10):
+                launch()
+            b.record(stream)
+            b.synchronize()
+            ms = a.elapsed_time(b) / 10
+            gbs = n * (plane_bytes + 320.0) / (ms * 1e-3) / 1e9
+            rates[f"{kind}_{name}"] = {"ms_per_launch": round(ms, 4), "images_per_s": round(n / (ms * 1e-3), 1),
+                                       "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log(f"[bench]   rates {rates}")
+    # errors (u8 input; the f32 copy of the same images gives the same planes)
+    D = dct64_matrix(torch, dev)
+    absD = D.abs()
+    ex_h, _, _, ex_p, ex_b = yume_pdq.hash_batch(u8, want_pdqf=True, want_buf1=True)
+    C = ex_b[:, :127, :127].to(torch.float64)
+    P64 = torch.matmul(torch.matmul(D, C), D.T)
+    M = torch.matmul(torch.matmul(absD, C.abs()), absD.T)
+    cmax = C.abs().amax(dim=(1, 2), keepdim=True)
+
+    def err_stats(p):
+        e = (p.to(torch.float64) - P64).abs()
+        return {"max_abs": float(e.max()), "rms_abs": float(e.pow(2).mean().sqrt()),
+                "max_rel_to_abs_product": float((e / M).max()),
+                "max_abs_per_unit_input": float((e / cmax).max())}
+
+    errors = {"exact": dict(err_stats(ex_p), hash_bits_vs_exact_mean=0.0, hash_bits_vs_exact_max=0,
+                            identical_hash_fraction=1.0)}
+    popc = torch.tensor([bin(v).count("1") for v in range(256)], dtype=torch.int32, device=dev)
+    for name, dct, layout in variants[3:]:
+        h, _, _, p = yume_pdq.hash_batch(u8, want_pdqf=True, dct=dct, layout=layout)
+        bits = popc[(h ^ ex_h).to(torch.int64)].sum(-1)
+        errors[name] = dict(err_stats(p), hash_bits_vs_exact_mean=round(float(bits.float().mean()), 4),
+                            hash_bits_vs_exact_max=int(bits.max()),
+                            identical_hash_fraction=round(float((bits == 0).float().mean()), 5))
+    log(f"[bench]   errors {errors}")
+    return {
+        "images": n,
+        "input": "u8 planes and the same images as f32 planes, resident in HBM",
+        "rates": rates,
+        "dct_error_vs_fp64": errors,
+        "reference_tolerances": {
+            "dct_abs_per_unit_input": "32 * f32 epsilon = 3.81e-06 (src/kernel/mod.rs:1239-1246, inputs in [-1, 1])",
+            "hash_bits": "16 bits per hash vs the pdqhash library (src/lib.rs:570-574)",
+        },
+        "note": ("errors: PDQF of every orientation 0 (NORMAL) against D64 C D64^T of the exact "
+                 "compressed plane C, FP64 on the GPU; max_abs_per_unit_input divides by max|C| of "
+                 "each image, the scale of the reference's DCT tolerance test; hash bits over all 8 "
+                 "orientations of every image against the exact kernel, which equals the reference's "
+                 "AVX2 kernel bit for bit"),
+    }
 
 
 def drop_in_ffi_us(yume_pdq, image, n=300):

@@ -96,22 +96,26 @@ __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, u
 }
 
 // Maximum over one 16-row tile of the dot products with every query tile of
-// the wave.  The products are integers in [-128, 128] and the bound is
-// positive, so comparing float bit patterns as signed integers orders them
-// exactly (negative floats are negative integers); integer max3 needs no NaN
+// the wave, kept in four running maxima (query tile t feeds mx[t % 4]) so the
+// max instructions of consecutive products do not wait on each other.  The
+// products are integers in [-128, 128] and the bound is positive, so
+// comparing float bit patterns as signed integers orders them exactly
+// (negative floats are negative integers); integer max3 needs no NaN
 // canonicalisation.  (Element-wise __float_as_int: a bit cast of a vector
 // element expression is miscompiled to element 0 by this toolchain.)
+constexpr int kMaxChains = 4;
+
 template <int QT>
-__device__ __forceinline__ int tile_max(const Expanded (&qa)[QT], const Expanded& rb, int mx) {
+__device__ __forceinline__ void tile_max(const Expanded (&qa)[QT], const Expanded& rb, int (&mx)[kMaxChains]) {
 #pragma unroll
   for (int t = 0; t < QT; t++) {
     const v4f c = mfma_prefix(qa[t], rb);
     const int c0 = __float_as_int(c.x), c1 = __float_as_int(c.y);
     const int c2 = __float_as_int(c.z), c3 = __float_as_int(c.w);
-    mx = max(max(mx, c0), c1);
-    mx = max(max(mx, c2), c3);
+    int& m = mx[t % kMaxChains];
+    m = max(max(m, c0), c1);
+    m = max(max(m, c2), c3);
   }
-  return mx;
 }
 
 // The rare path: recompute one tile's products and re-check every pair that
@@ -146,9 +150,11 @@ struct RowSource {
 };
 
 // Stream the wave's tiles: loads run kAhead tiles ahead, the next tile's
-// operand is expanded while the current tile's products run, and the
-// bound is tested once per kGroup tiles (a wave-uniform branch that is taken
-// about once per million tiles on random data).
+// operand is expanded while the current tile's products run, and the bound is
+// tested once per kGroup tiles.  A group that reaches it (about once per
+// million tiles on random data, plus the true hits) is only noted in a
+// wave-uniform bit mask; its tiles are recomputed and checked pair by pair
+// after the stream, so the loop itself holds no call and no branch.
 template <int QT, bool kFull, bool kPlane>
 __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uint32_t* __restrict__ lut,
                                              const RowSource<kPlane>& src, const uint32_t* __restrict__ db,
@@ -159,6 +165,7 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
   constexpr int kAhead = 8;
   constexpr int kGroup = kFull ? 4 : 1;
   static_assert(kMfmaTilesPerWave % kAhead == 0 && kAhead % kGroup == 0, "tile loop shape");
+  static_assert(kMfmaTilesPerWave / kGroup <= 64, "one mask bit per group");
   // rows of this wave that exist (the last wave of the database only)
   const uint32_t left = kFull ? 0u : (uint32_t)(n_rows - row0);
   const uint32_t tiles = kFull ? (uint32_t)kMfmaTilesPerWave : (left + 15) / 16;
@@ -171,16 +178,17 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
   for (int a = 0; a < kAhead; a++) ring[a] = fetch(a < (int)tiles ? a : tiles - 1);
   Expanded cur;
   expand_word(ring[0], lut, cur);
-  uint32_t held[kGroup];  // raw words of the tiles since the last test
-  int mx = INT32_MIN;
+  int mx[kMaxChains];
+#pragma unroll
+  for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
   const int ibound = __float_as_int(bound);
+  uint64_t pending = 0;  // groups whose products reached the bound
 #pragma unroll 1
   for (uint32_t tile = 0; tile < tiles; tile += kAhead) {
 #pragma unroll
     for (int a = 0; a < kAhead; a++) {
       const uint32_t t = tile + a;
       if (!kFull && t >= tiles) break;
-      held[a % kGroup] = ring[a];
       // Always issue the load (the last kAhead tiles re-read tile `tiles - 1`
       // and drop it): a load under a branch makes the wait counters unknown
       // to the compiler, which then waits for every load in flight.
@@ -188,20 +196,28 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
       Expanded rb = cur;
       if (!kFull && t * 16 + rn >= left) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
       expand_word(ring[(a + 1) % kAhead], lut, cur);  // operand of tile t + 1
-      mx = tile_max<QT>(qa, rb, mx);
+      tile_max<QT>(qa, rb, mx);
       if (a % kGroup == kGroup - 1) {
-        if (__builtin_expect(wave_any(mx >= ibound), 0)) {
-          for (int g = 0; g < kGroup; g++) {
-            const uint32_t tg = t + 1 - kGroup + g;
-            Expanded eg;
-            expand_word(held[g], lut, eg);
-            if (!kFull && tg * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
-            resolve_tile<QT>(qa, eg, row0 + (uint64_t)tg * 16, qbase, rn, kq, bound, queries, db, n_queries, n_rows,
-                             threshold, hits, capacity, counter, index_base);
-          }
-        }
-        mx = INT32_MIN;
+        const int mall = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
+        pending |= (uint64_t)(wave_any(mall >= ibound) ? 1u : 0u) << (t / kGroup);
+#pragma unroll
+        for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
       }
+      // keep each tile's load, operand expansion and products together: the
+      // scheduler would otherwise gather the eight loads of an iteration at
+      // its end behind a wait for every load in flight
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  while (pending) {
+    const uint32_t g = (uint32_t)__builtin_ctzll(pending);
+    pending &= pending - 1;
+    for (uint32_t tg = g * kGroup; tg < (g + 1) * kGroup; tg++) {
+      Expanded eg;
+      expand_word(fetch(tg), lut, eg);
+      if (!kFull && tg * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
+      resolve_tile<QT>(qa, eg, row0 + (uint64_t)tg * 16, qbase, rn, kq, bound, queries, db, n_queries, n_rows,
+                       threshold, hits, capacity, counter, index_base);
     }
   }
 }
@@ -271,7 +287,7 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
 
 // One launch, one block per work item (row chunks x query groups).
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
     uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
@@ -291,7 +307,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // serves batches of any size; batches of fewer than min_nq queries are left
 // to the vector-ALU kernel and this one returns at once.
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 2 : 4))) pdq_match_mfma_shards_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_shards_kernel(
     const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
     const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t min_nq,
     uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter) {
@@ -414,12 +430,17 @@ hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* plane, u
   return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
 }
 
+#ifndef YPDQ_DEFAULT_QT
+#define YPDQ_DEFAULT_QT 8
+#endif
+constexpr int kDefaultQt = YPDQ_DEFAULT_QT;
+
 hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                              unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
   // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
   // 1024 / 8192 queries); larger tiles cost occupancy
-  return launch_match_mfma_qt(8, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
+  return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
                               index_base, stream);
 }
 
