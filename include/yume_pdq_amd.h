@@ -27,7 +27,9 @@ enum {
   YUME_PDQ_AMD_ENOMEM = -3,    /* device allocation failed */
   YUME_PDQ_AMD_ENODEV = -4,    /* no usable GPU */
   YUME_PDQ_AMD_EOVERFLOW = -5, /* more hits than the buffer holds (count still exact) */
-  YUME_PDQ_AMD_ETIMEOUT = -6   /* a wait ran out of time */
+  YUME_PDQ_AMD_ETIMEOUT = -6,  /* a wait ran out of time */
+  YUME_PDQ_AMD_EIO = -7,       /* a read or write of the stream failed */
+  YUME_PDQ_AMD_EEOF = -8       /* the input stream ended inside a frame */
 };
 
 /* Input planes: f32 luma (0..255), u8 luma, or 8-bit colour converted to luma
@@ -387,6 +389,47 @@ int yume_pdq_amd_server_wait(YumePdqAmdServer *server, uint64_t request_id, int 
                              YumePdqAmdResult *out);
 int yume_pdq_amd_server_release(YumePdqAmdServer *server, uint64_t request_id);
 int yume_pdq_amd_server_stats(const YumePdqAmdServer *server, YumePdqAmdServerStats *out);
+
+/*
+ * Stream ingestion (the reference's `pipe` subcommand, src/main.rs:1091-1304).
+ *
+ * Reads raw 512x512 frames from input_fd until the end of the stream, hashes
+ * them on the GPU in batches of batch_frames, and writes one record per frame
+ * (or eight with dihedrals: the original, then visit_dihedrals' order) to
+ * output_fd in `output_format`, the reference's -f spelling:
+ *   raw | q+raw | hex | HEX | q+hex | q+HEX | bin | q+bin | diagnostic,
+ *   optionally followed by +nl | +lf | +crlf | +cr | +nul (a separator after
+ *   every record).
+ * input_kind: YUME_PDQ_AMD_INPUT_U8 (luma8), _RGB8 or _RGBA8 (BT.601 luma).
+ * dihedral_mode: YUME_PDQ_AMD_DIHEDRAL_AVX2 reproduces the x86 reference
+ * binary.  Returns OK at a clean end of input, YUME_PDQ_AMD_EEOF after
+ * writing every whole frame when the input ends inside a frame,
+ * YUME_PDQ_AMD_EIO on a read/write error, EINVAL for a bad format.
+ * *frames: frames written (may be NULL).
+ */
+typedef struct YumePdqAmdPipeConfig {
+  int input_fd;
+  int output_fd;
+  int input_kind;
+  const char *output_format;
+  int dihedrals;
+  uint32_t dihedral_mode;
+  uint32_t batch_frames; /* 0 = 64, at most 4096 */
+  int stats;             /* progress lines on stderr about once a second */
+} YumePdqAmdPipeConfig;
+int yume_pdq_amd_pipe_run(const YumePdqAmdPipeConfig *cfg, uint64_t *frames);
+
+/* Host-only formatting of hashed frames in a pipe output format (no GPU
+ * call).  hashes n x 8 x 32, quality and threshold n x 8 as
+ * yume_pdq_amd_hash_batch returns them; dihedral_pdqf n x 8 x 256 for the
+ * diagnostic format (else NULL).  Writes at most out_capacity bytes and
+ * returns the bytes written, or a negative error (EINVAL: bad format or
+ * missing PDQFs; EOVERFLOW: out_capacity too small).
+ * yume_pdq_amd_format_bound gives a sufficient capacity (0: bad format). */
+int64_t yume_pdq_amd_format_frames(const char *output_format, uint32_t n_frames, int dihedrals,
+                                   const uint8_t *hashes, const float *quality, const float *threshold,
+                                   const float *dihedral_pdqf, char *out, uint64_t out_capacity);
+uint64_t yume_pdq_amd_format_bound(const char *output_format, uint32_t n_frames, int dihedrals);
 
 #ifdef __cplusplus
 } /* extern "C" */

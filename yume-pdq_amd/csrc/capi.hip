@@ -7,11 +7,13 @@
 #include <string.h>
 
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/yume_pdq.h"
 #include "../../include/yume_pdq_amd.h"
 #include "pdq_device.h"
+#include "pipe.h"
 
 // ---------------------------------------------------------------------------
 // Data symbols of the reference ABI.
@@ -200,6 +202,8 @@ const char* yume_pdq_amd_strerror(int code) {
     case YUME_PDQ_AMD_ENODEV: return "no usable GPU";
     case YUME_PDQ_AMD_EOVERFLOW: return "hit buffer overflow";
     case YUME_PDQ_AMD_ETIMEOUT: return "timed out";
+    case YUME_PDQ_AMD_EIO: return "stream read or write failed";
+    case YUME_PDQ_AMD_EEOF: return "input ended inside a frame";
     default: return "unknown error";
   }
 }
@@ -438,3 +442,55 @@ int yume_pdq_amd_stage_luma(const uint8_t* d_pixels, int channels, uint64_t n_pi
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stream ingestion and pipe output formats
+// ---------------------------------------------------------------------------
+int yume_pdq_amd_pipe_run(const YumePdqAmdPipeConfig* cfg, uint64_t* frames) {
+  if (frames) *frames = 0;
+  if (!cfg) return YUME_PDQ_AMD_EINVAL;
+  ypdq::PipeOptions o;
+  if (!ypdq::parse_output_format(cfg->output_format, &o.format)) return YUME_PDQ_AMD_EINVAL;
+  if (cfg->input_kind != YUME_PDQ_AMD_INPUT_U8 && cfg->input_kind != YUME_PDQ_AMD_INPUT_RGB8 &&
+      cfg->input_kind != YUME_PDQ_AMD_INPUT_RGBA8)
+    return YUME_PDQ_AMD_EINVAL;
+  if (cfg->dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2 || cfg->input_fd < 0 || cfg->output_fd < 0)
+    return YUME_PDQ_AMD_EINVAL;
+  o.input_fd = cfg->input_fd;
+  o.output_fd = cfg->output_fd;
+  o.input_kind = cfg->input_kind;
+  o.dihedrals = cfg->dihedrals != 0;
+  o.dihedral_mode = cfg->dihedral_mode;
+  o.batch = cfg->batch_frames ? cfg->batch_frames : 64;
+  if (o.batch > 4096) return YUME_PDQ_AMD_EINVAL;
+  o.stats = cfg->stats != 0;
+  std::string err;
+  const int rc = ypdq::run_pipe(o, frames, &err);
+  if (rc == 0) return YUME_PDQ_AMD_OK;
+  if (!err.empty()) fprintf(stderr, "yume_pdq_amd_pipe_run: %s\n", err.c_str());
+  if (rc == ypdq::kPipeEio) return YUME_PDQ_AMD_EIO;
+  if (rc == ypdq::kPipeEof) return YUME_PDQ_AMD_EEOF;
+  if (rc == -1) return YUME_PDQ_AMD_EINVAL;
+  return YUME_PDQ_AMD_EHIP;
+}
+
+int64_t yume_pdq_amd_format_frames(const char* output_format, uint32_t n_frames, int dihedrals, const uint8_t* hashes,
+                                   const float* quality, const float* threshold, const float* dihedral_pdqf, char* out,
+                                   uint64_t out_capacity) {
+  ypdq::OutputFormat f;
+  if (!ypdq::parse_output_format(output_format, &f)) return YUME_PDQ_AMD_EINVAL;
+  if (n_frames == 0) return 0;
+  if (!hashes || !quality || !threshold || (!out && out_capacity)) return YUME_PDQ_AMD_EINVAL;
+  if (f.type == ypdq::OutputFormat::kDiagnostic && !dihedral_pdqf) return YUME_PDQ_AMD_EINVAL;
+  std::string text;
+  ypdq::format_frames(f, n_frames, dihedrals != 0, hashes, quality, threshold, dihedral_pdqf, text);
+  if (text.size() > out_capacity) return YUME_PDQ_AMD_EOVERFLOW;
+  if (!text.empty()) memcpy(out, text.data(), text.size());
+  return (int64_t)text.size();
+}
+
+uint64_t yume_pdq_amd_format_bound(const char* output_format, uint32_t n_frames, int dihedrals) {
+  ypdq::OutputFormat f;
+  if (!ypdq::parse_output_format(output_format, &f)) return 0;
+  return ypdq::format_bound(f, n_frames, dihedrals != 0);
+}

@@ -73,6 +73,12 @@ class HashOptions(C.Structure):
                 ("luma_coeff", C.POINTER(C.c_float))]
 
 
+class PipeConfig(C.Structure):
+    _fields_ = [("input_fd", C.c_int), ("output_fd", C.c_int), ("input_kind", C.c_int),
+                ("output_format", C.c_char_p), ("dihedrals", C.c_int), ("dihedral_mode", C.c_uint32),
+                ("batch_frames", C.c_uint32), ("stats", C.c_int)]
+
+
 class TuneRecord(C.Structure):
     _fields_ = [("n_queries", C.c_uint32), ("path", C.c_uint32), ("query_tiles", C.c_uint32), ("ms", C.c_float)]
 
@@ -98,6 +104,12 @@ def lib():
         L.yume_pdq_amd_device_info.argtypes = [i, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
         L.yume_pdq_amd_hash_batch.restype = i
         L.yume_pdq_amd_hash_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_pipe_run.restype = i
+        L.yume_pdq_amd_pipe_run.argtypes = [C.POINTER(PipeConfig), C.POINTER(C.c_uint64)]
+        L.yume_pdq_amd_format_frames.restype = C.c_int64
+        L.yume_pdq_amd_format_frames.argtypes = [C.c_char_p, u32, i, vp, vp, vp, vp, vp, u64]
+        L.yume_pdq_amd_format_bound.restype = u64
+        L.yume_pdq_amd_format_bound.argtypes = [C.c_char_p, u32, i]
         L.yume_pdq_amd_hash_batch_ex.restype = i
         L.yume_pdq_amd_hash_batch_ex.argtypes = [vp, i, u64, u32, C.POINTER(HashOptions), vp, vp, vp, vp, vp, vp]
         L.yume_pdq_amd_hash_batch_luma.restype = i
@@ -778,3 +790,64 @@ class Server:
         st = ServerStats()
         _check(lib().yume_pdq_amd_server_stats(self._h, C.byref(st)), "server_stats")
         return {"contexts": st.contexts, "device_bytes": st.device_bytes, "batches": st.batches, "images": st.images}
+
+
+# ---------------------------------------------------------------------------
+# Stream ingestion (the reference's `pipe` subcommand) and its output formats
+# ---------------------------------------------------------------------------
+PIPE_INPUT_FORMATS = {"luma8": INPUT_U8, "LUMA8": INPUT_U8, "rgb8": INPUT_RGB8, "RGB8": INPUT_RGB8,
+                      "rgba8": INPUT_RGBA8, "RGBA8": INPUT_RGBA8}
+
+
+def format_frames(fmt, hashes, quality, threshold, dihedrals=False, dihedral_pdqf=None):
+    """Records of hashed frames in a pipe output format (host-only): hashes
+    (n, 8, 32) u8, quality and threshold (n, 8) f32 as hash_batch returns them,
+    dihedral_pdqf (n, 8, 16, 16) f32 for the diagnostic format.  Returns bytes."""
+    h = np.ascontiguousarray(hashes, np.uint8).reshape(-1, 8, 32)
+    q = np.ascontiguousarray(quality, np.float32).reshape(-1, 8)
+    t = np.ascontiguousarray(threshold, np.float32).reshape(-1, 8)
+    n = h.shape[0]
+    p = None if dihedral_pdqf is None else np.ascontiguousarray(dihedral_pdqf, np.float32).reshape(n, 8, 256)
+    f = fmt.encode()
+    cap = int(lib().yume_pdq_amd_format_bound(f, n, int(bool(dihedrals))))
+    if cap == 0 and n > 0:
+        raise YumePdqError(f"invalid output format: {fmt!r}")
+    buf = C.create_string_buffer(max(cap, 1))
+    got = _check(lib().yume_pdq_amd_format_frames(
+        f, n, int(bool(dihedrals)), h.ctypes.data, q.ctypes.data, t.ctypes.data,
+        p.ctypes.data if p is not None else None, buf, cap), "format_frames")
+    return buf.raw[:got]
+
+
+def pipe(source, dest, fmt="bin", input_format="luma8", dihedrals=False, batch=64, stats=False,
+         dihedral_mode=DIHEDRAL_AVX2):
+    """Hash every whole 512x512 frame of `source` on the GPU and write the
+    records to `dest` (paths or open file descriptors), as the reference's
+    `yume-pdq pipe -f fmt [--dihedrals] --input-format ...`.  Returns the
+    number of frames; raises YumePdqError (EEOF after writing every whole
+    frame if the input ends inside a frame)."""
+    if input_format not in PIPE_INPUT_FORMATS:
+        raise ValueError(f"invalid input format: {input_format!r}")
+    own = []
+    try:
+        if isinstance(source, int):
+            fd_in = source
+        else:
+            fd_in = os.open(source, os.O_RDONLY)
+            own.append(fd_in)
+        if isinstance(dest, int):
+            fd_out = dest
+        else:
+            fd_out = os.open(dest, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+            own.append(fd_out)
+        cfg = PipeConfig(fd_in, fd_out, PIPE_INPUT_FORMATS[input_format], fmt.encode(), int(bool(dihedrals)),
+                         dihedral_mode, batch, int(bool(stats)))
+        frames = C.c_uint64(0)
+        rc = lib().yume_pdq_amd_pipe_run(C.byref(cfg), C.byref(frames))
+        if rc < 0:
+            raise YumePdqError(f"pipe: {lib().yume_pdq_amd_strerror(rc).decode()} ({rc}) after "
+                               f"{frames.value} frames")
+        return frames.value
+    finally:
+        for fd in own:
+            os.close(fd)
