@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-hash-precisions", action="store_true",
                     help="skip the batched-hashing section (4096 images, every DCT precision)")
     ap.add_argument("--hash-images", type=int, default=4096)
+    ap.add_argument("--no-pipe", action="store_true", help="skip the stream-ingestion section")
+    ap.add_argument("--pipe-frames", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-offered-10m", type=float, default=60_000.0,
                     help="offered images/s of the 10M-row serving run")
@@ -304,6 +306,10 @@ def main():
     if not args.no_hash_precisions:
         precisions = hash_precision_section(args, yume_pdq, torch, dev, synthetic, planes_u8)
 
+    pipe_stats = None
+    if not args.no_pipe:
+        pipe_stats = pipe_section(args, yume_pdq, torch, planes_u8)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, db, thr, n_rows)
@@ -350,6 +356,7 @@ def main():
         "serving": serving,
         "uneven_batching": uneven,
         "batched_hashing": precisions,
+        "stream_ingestion": pipe_stats,
         "drop_in_ffi": {
             "us_per_hash": round(ffi_us, 2),
             "note": ("PCIe-inclusive per-image rate of the reference ABI: 1 MiB f32 host image "
@@ -628,6 +635,44 @@ def hash_precision_section(args, yume_pdq, torch, dev, synthetic, planes_u8):
                  "orientations of every image against the exact kernel, which equals the reference's "
                  "AVX2 kernel bit for bit"),
     }
+
+
+def pipe_section(args, yume_pdq, torch, planes_u8):
+    """Stream ingestion (the reference's `pipe`): a file of raw luma8 frames
+    in the page cache, hashed in GPU batches and written as text records to
+    /dev/null; frames/s of the whole call (reader thread, uploads, kernel,
+    formatting, writes)."""
+    import tempfile
+    n = args.pipe_frames
+    pool = planes_u8[:64].cpu().numpy().tobytes()
+    reps = (n + 63) // 64
+    fd, path = tempfile.mkstemp(prefix="pipe_frames_", suffix=".raw")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            for _ in range(reps):
+                f.write(pool)
+        frames = reps * 64
+        out = {}
+        for label, fmt, dih, batch in (("hex+lf", "hex+lf", False, 64), ("hex+lf_batch256", "hex+lf", False, 256),
+                                       ("q+hex+lf_dihedrals", "q+hex+lf", True, 64), ("bin", "bin", False, 64)):
+            dn = os.open(os.devnull, os.O_WRONLY)
+            try:
+                yume_pdq.pipe(path, dn, fmt=fmt, dihedrals=dih, batch=batch)  # warm-up
+                t0 = time.perf_counter()
+                got = yume_pdq.pipe(path, dn, fmt=fmt, dihedrals=dih, batch=batch)
+                dt = time.perf_counter() - t0
+            finally:
+                os.close(dn)
+            assert got == frames
+            out[label] = {"frames_per_s": round(frames / dt, 1), "seconds": round(dt, 4), "batch": batch,
+                          "records_per_frame": 8 if dih else 1}
+        log(f"[bench]   pipe {out}")
+        return {"frames": frames, "input": "luma8 512x512 frames from a file in the page cache",
+                "output": "/dev/null", "rates": out,
+                "note": ("one yume_pdq_amd_pipe_run call per line, setup (pinned and device buffers) "
+                         "included; the reference's pipe hashes on two CPU threads")}
+    finally:
+        os.unlink(path)
 
 
 def drop_in_ffi_us(yume_pdq, image, n=300):
