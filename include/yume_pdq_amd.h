@@ -147,6 +147,22 @@ int yume_pdq_amd_hash_batch_ex(const void *d_planes, int input_kind, uint64_t pl
                                void *stream);
 
 /*
+ * The precise kernel, hash_get_threshold with ReferenceKernel<f32>
+ * (src/kernel/mod.rs:691-821, src/kernel/ref.rs): two passes of a 3-wide box
+ * filter along rows and columns, sampling at floor((2i + 1) 512 / 254), the
+ * sequential DCT, Torben's median.  The NORMAL orientation only.
+ * input_kind: YUME_PDQ_AMD_INPUT_F32 or _U8.  d_workspace: at least
+ * yume_pdq_amd_hash_ref_workspace_bytes(n_images) bytes of device memory
+ * (2 MiB per image).  d_hashes n x 32, d_quality and d_threshold n floats,
+ * d_pdqf n x 256 floats or NULL.
+ */
+uint64_t yume_pdq_amd_hash_ref_workspace_bytes(uint32_t n_images);
+int yume_pdq_amd_hash_ref_batch(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
+                                uint32_t n_images, void *d_workspace, uint64_t workspace_bytes,
+                                uint8_t *d_hashes, float *d_quality, float *d_threshold, float *d_pdqf,
+                                void *stream);
+
+/*
  * Batched visit_dihedrals (src/lib.rs:358-420) from given PDQFs.
  * d_pdqf: n x 256; d_thr_in: n thresholds (as returned with each PDQF).
  * Outputs as yume_pdq_amd_hash_batch (entry 0 = the NORMAL orientation);

@@ -302,6 +302,23 @@ int yume_pdq_amd_hash_batch_ex(const void* d_planes, int input_kind, uint64_t pl
                                           o.dihedral_mode, (int)o.dct_precision, waves, out, o.luma_coeff, s));
 }
 
+uint64_t yume_pdq_amd_hash_ref_workspace_bytes(uint32_t n_images) { return ypdq::hash_ref_workspace_bytes(n_images); }
+
+int yume_pdq_amd_hash_ref_batch(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
+                                void* d_workspace, uint64_t workspace_bytes, uint8_t* d_hashes, float* d_quality,
+                                float* d_threshold, float* d_pdqf, void* stream) {
+  if (n_images == 0) return YUME_PDQ_AMD_OK;
+  if (!d_planes || !d_workspace || !d_hashes || !d_quality || !d_threshold) return YUME_PDQ_AMD_EINVAL;
+  if (input_kind != YUME_PDQ_AMD_INPUT_F32 && input_kind != YUME_PDQ_AMD_INPUT_U8) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
+  if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
+    return YUME_PDQ_AMD_EINVAL;
+  if (workspace_bytes < ypdq::hash_ref_workspace_bytes(n_images) || (uintptr_t)d_workspace % 16)
+    return YUME_PDQ_AMD_EINVAL;
+  return from_hip(ypdq::launch_hash_ref(d_planes, input_kind, plane_stride_bytes, n_images, d_workspace, d_hashes,
+                                        d_quality, d_threshold, d_pdqf, static_cast<hipStream_t>(stream)));
+}
+
 int yume_pdq_amd_dihedrals_batch(const float* d_pdqf, const float* d_thr_in, uint32_t n, uint32_t dihedral_mode,
                                  uint8_t* d_hashes, float* d_quality, float* d_threshold, float* d_dihedral_pdqf,
                                  void* stream) {

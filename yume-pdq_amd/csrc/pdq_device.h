@@ -75,6 +75,12 @@ hipError_t launch_hash_block(const void* planes, int input_kind, uint64_t plane_
 // Layout the automatic choice uses for n images: 1 = wave per image,
 // 8 = block of eight waves per image.
 int hash_layout_for(uint32_t n_images, int dct);
+// The precise ReferenceKernel<f32> (f32 or u8 planes): hash, quality,
+// threshold (the median) and PDQF of the NORMAL orientation per image;
+// `work` holds hash_ref_workspace_bytes(n) bytes of scratch.
+uint64_t hash_ref_workspace_bytes(uint32_t n_images);
+hipError_t launch_hash_ref(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, void* work,
+                           uint8_t* hashes, float* quality, float* threshold, float* pdqf, hipStream_t stream);
 hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, uint32_t mode,
                             const HashOutputs& out, hipStream_t stream);
 // Shape of a matcher launch: rows per lane per iteration, threads per block,

@@ -104,6 +104,10 @@ def lib():
         L.yume_pdq_amd_device_info.argtypes = [i, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
         L.yume_pdq_amd_hash_batch.restype = i
         L.yume_pdq_amd_hash_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, vp, vp, vp]
+        L.yume_pdq_amd_hash_ref_workspace_bytes.restype = u64
+        L.yume_pdq_amd_hash_ref_workspace_bytes.argtypes = [u32]
+        L.yume_pdq_amd_hash_ref_batch.restype = i
+        L.yume_pdq_amd_hash_ref_batch.argtypes = [vp, i, u64, u32, vp, u64, vp, vp, vp, vp, vp]
         L.yume_pdq_amd_pipe_run.restype = i
         L.yume_pdq_amd_pipe_run.argtypes = [C.POINTER(PipeConfig), C.POINTER(C.c_uint64)]
         L.yume_pdq_amd_format_frames.restype = C.c_int64
@@ -851,3 +855,36 @@ def pipe(source, dest, fmt="bin", input_format="luma8", dihedrals=False, batch=6
     finally:
         for fd in own:
             os.close(fd)
+
+
+def hash_ref_batch(planes, want_pdqf=False, stream=None):
+    """The precise ReferenceKernel<f32> (Jarosz box filter resampling) on n
+    512x512 f32 or u8 planes resident on the GPU: NORMAL orientation only.
+    Returns (hashes (n,32) u8, quality (n,) f32, threshold (n,) f32[, pdqf (n,16,16)])."""
+    torch = _torch()
+    if planes.dim() == 2:
+        planes = planes.unsqueeze(0)
+    if not planes.is_cuda or planes.shape[1:] != (512, 512):
+        raise ValueError("planes must be a CUDA tensor of shape (n, 512, 512)")
+    if planes.dtype == torch.float32:
+        kind, elem = INPUT_F32, 4
+    elif planes.dtype == torch.uint8:
+        kind, elem = INPUT_U8, 1
+    else:
+        raise ValueError("planes must be float32 or uint8")
+    planes = planes.contiguous()
+    n = planes.shape[0]
+    dev = planes.device
+    ws_bytes = int(lib().yume_pdq_amd_hash_ref_workspace_bytes(n))
+    ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.float32, device=dev)
+    hashes = torch.empty((n, 32), dtype=torch.uint8, device=dev)
+    quality = torch.empty(n, dtype=torch.float32, device=dev)
+    threshold = torch.empty(n, dtype=torch.float32, device=dev)
+    pdqf = torch.empty((n, 16, 16), dtype=torch.float32, device=dev) if want_pdqf else None
+    _check(lib().yume_pdq_amd_hash_ref_batch(
+        C.c_void_p(planes.data_ptr()), kind, 512 * 512 * elem, n, C.c_void_p(ws.data_ptr()), ws_bytes,
+        C.c_void_p(hashes.data_ptr()), C.c_void_p(quality.data_ptr()), C.c_void_p(threshold.data_ptr()),
+        C.c_void_p(pdqf.data_ptr() if pdqf is not None else 0), _stream_ptr(stream)), "hash_ref_batch")
+    if want_pdqf:
+        return hashes, quality, threshold, pdqf
+    return hashes, quality, threshold
