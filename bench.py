@@ -594,6 +594,34 @@ def hash_precision_section(args, yume_pdq, torch, dev, synthetic, planes_u8):
             gbs = n * (plane_bytes + 320.0) / (ms * 1e-3) / 1e9
             rates[f"{kind}_{name}"] = {"ms_per_launch": round(ms, 4), "images_per_s": round(n / (ms * 1e-3), 1),
                                        "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # the precise ReferenceKernel<f32> (box-filter resampling), u8 planes
+    n_ref = min(n, 1024)
+    ws = torch.empty(int(yume_pdq.lib().yume_pdq_amd_hash_ref_workspace_bytes(n_ref)) // 4, dtype=torch.float32,
+                     device=dev)
+    ref_out = (torch.empty((n_ref, 32), dtype=torch.uint8, device=dev),
+               torch.empty(n_ref, dtype=torch.float32, device=dev), torch.empty(n_ref, dtype=torch.float32, device=dev))
+    import ctypes
+
+    def launch_ref():
+        rc = yume_pdq.lib().yume_pdq_amd_hash_ref_batch(
+            ctypes.c_void_p(u8.data_ptr()), yume_pdq.INPUT_U8, 512 * 512, n_ref, ctypes.c_void_p(ws.data_ptr()),
+            ws.numel() * 4, ctypes.c_void_p(ref_out[0].data_ptr()), ctypes.c_void_p(ref_out[1].data_ptr()),
+            ctypes.c_void_p(ref_out[2].data_ptr()), None, ctypes.c_void_p(stream.cuda_stream))
+        assert rc == 0
+    launch_ref()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(5):
+        launch_ref()
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / 5
+    rates["u8_reference_kernel"] = {"ms_per_launch": round(ms, 4), "images": n_ref,
+                                    "images_per_s": round(n_ref / (ms * 1e-3), 1),
+                                    "note": ("ReferenceKernel<f32>: four sequential box passes through "
+                                             "2 MiB of scratch per image, NORMAL orientation only")}
+    del ws
     log(f"[bench]   rates {rates}")
     # errors (u8 input; the f32 copy of the same images gives the same planes)
     D = dct64_matrix(torch, dev)
