@@ -149,9 +149,9 @@ struct RowSource {
   }
 };
 
-// Stream the wave's tiles: loads run kAhead tiles ahead, the next tile's
-// operand is expanded while the current tile's products run, and the bound is
-// tested once per kGroup tiles.  A group that reaches it (about once per
+// Stream the wave's tiles: loads run kAhead tiles ahead, each tile's word is
+// expanded to fp4 codes just before its products, and the bound is tested
+// once per kGroup tiles.  A group that reaches it (about once per
 // million tiles on random data, plus the true hits) is only noted in a
 // wave-uniform bit mask; its tiles are recomputed and checked pair by pair
 // after the stream, so the loop itself holds no call and no branch.
@@ -176,8 +176,6 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
   uint32_t ring[kAhead];
 #pragma unroll
   for (int a = 0; a < kAhead; a++) ring[a] = fetch(a < (int)tiles ? a : tiles - 1);
-  Expanded cur;
-  expand_word(ring[0], lut, cur);
   int mx[kMaxChains];
 #pragma unroll
   for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
@@ -189,13 +187,13 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
     for (int a = 0; a < kAhead; a++) {
       const uint32_t t = tile + a;
       if (!kFull && t >= tiles) break;
+      Expanded rb;
+      expand_word(ring[a], lut, rb);
+      if (!kFull && t * 16 + rn >= left) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
       // Always issue the load (the last kAhead tiles re-read tile `tiles - 1`
       // and drop it): a load under a branch makes the wait counters unknown
       // to the compiler, which then waits for every load in flight.
       ring[a] = fetch(t + kAhead < tiles ? t + kAhead : tiles - 1);
-      Expanded rb = cur;
-      if (!kFull && t * 16 + rn >= left) rb.d[0] = rb.d[1] = rb.d[2] = rb.d[3] = 0;
-      expand_word(ring[(a + 1) % kAhead], lut, cur);  // operand of tile t + 1
       tile_max<QT>(qa, rb, mx);
       if (a % kGroup == kGroup - 1) {
         const int mall = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
@@ -203,10 +201,6 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
 #pragma unroll
         for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
       }
-      // keep each tile's load, operand expansion and products together: the
-      // scheduler would otherwise gather the eight loads of an iteration at
-      // its end behind a wait for every load in flight
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
   while (pending) {
@@ -287,7 +281,7 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
 
 // One launch, one block per work item (row chunks x query groups).
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
     uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
@@ -307,7 +301,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // serves batches of any size; batches of fewer than min_nq queries are left
 // to the vector-ALU kernel and this one returns at once.
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_shards_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_shards_kernel(
     const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
     const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t min_nq,
     uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter) {
