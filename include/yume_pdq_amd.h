@@ -372,6 +372,7 @@ typedef struct YumePdqAmdServerConfig {
   uint64_t hit_capacity;      /* per batch (a larger batch result is re-matched) */
   int use_graph;
   int busy_poll;              /* worker spins (yielding) while batches are in flight */
+  int in_order_compute;       /* batches' kernels run one batch after another (uploads overlap) */
 } YumePdqAmdServerConfig;
 typedef struct YumePdqAmdResult {
   int status;                 /* YUME_PDQ_AMD_OK or an error of the batch */

@@ -193,6 +193,7 @@ int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig
   c.hit_capacity = cfg->hit_capacity;
   c.use_graph = cfg->use_graph != 0;
   c.busy_poll = cfg->busy_poll != 0;
+  c.in_order_compute = cfg->in_order_compute != 0;
   const hipError_t e = v->server.start(s->set, c);
   if (e != hipSuccess) {
     delete v;

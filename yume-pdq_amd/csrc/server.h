@@ -28,6 +28,12 @@ struct ServerConfig {
   uint64_t hit_capacity = 1 << 20;  // per batch; a batch with more hits is re-matched into a larger buffer
   bool use_graph = true;
   bool busy_poll = true;  // spin (yielding) while batches are in flight instead of sleeping between polls
+  // Run the batches' kernels in launch order (each waits for the previous
+  // batch's kernels) rather than side by side, and upload each request as it
+  // is admitted: the next batch fills and uploads while the current one
+  // computes, and starts when it ends (or when it is full).  Concurrent
+  // batches would share the GPU and all finish late.
+  bool in_order_compute = true;
 };
 
 struct RequestResult {
@@ -76,7 +82,9 @@ class Server {
   struct Slot {  // one batch context and the requests it is running
     std::unique_ptr<BatchContext> ctx;
     hipEvent_t done = nullptr;
-    bool busy = false;
+    hipEvent_t computed = nullptr;  // the batch's kernels have finished
+    bool busy = false;     // launched: kernels and readback queued
+    bool open = false;     // admitting requests, their uploads queued
     uint64_t batch_no = 0;
     uint32_t n_images = 0;
     std::vector<Request> reqs;
@@ -88,6 +96,11 @@ class Server {
   };
   void loop();
   bool launch_batch(Slot& s);
+  // in-order mode: admit pending requests into the open slot (uploading
+  // each); true when the next pending request would not fit
+  bool admit(Slot& s, bool* admitted);
+  void launch_open(Slot& s);
+  void fail_batch(Slot& s);
   void harvest(Slot& s);
   void free_slots(uint64_t first, uint32_t n);
 
@@ -99,6 +112,8 @@ class Server {
   uint64_t ring_slots_ = 0;
   std::vector<uint8_t> slot_used_;
   uint64_t ring_head_ = 0;  // next slot to hand out
+  hipEvent_t last_computed_ = nullptr;  // kernels of the last launched batch (in-order compute)
+  int open_ = -1;                       // the slot admitting requests (in-order compute)
 
   std::mutex mu_;
   std::condition_variable cv_submit_;  // worker: new requests

@@ -52,6 +52,7 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
     // admission by HBM budget: a context is kept only if it fits
     if (device_bytes() + s->ctx->device_bytes() > cfg.hbm_budget_bytes) break;
     if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&s->computed, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out),
                            s->ctx->out_bytes(std::min<uint64_t>(kHitPrefix, cfg.hit_capacity)),
                            hipHostMallocDefault)) ||
@@ -82,6 +83,7 @@ void Server::stop() {
   if (worker_.joinable()) worker_.join();
   for (auto& s : ctx_) {
     if (s->done) (void)hipEventDestroy(s->done);
+    if (s->computed) (void)hipEventDestroy(s->computed);
     void* h[] = {s->h_out, s->h_hits};
     for (void* p : h)
       if (p) (void)hipHostFree(p);
@@ -184,21 +186,73 @@ bool Server::launch_batch(Slot& s) {
   s.n_images = total;
   s.batch_no = batches_.fetch_add(1) + 1;
   s.busy = true;
-  if (e != hipSuccess) {
-    // report the failure to every request of the batch
-    std::lock_guard<std::mutex> g(mu_);
-    for (auto& r : s.reqs) {
-      RequestResult rr;
-      rr.status = -2;
-      rr.n_images = r.n;
-      results_[r.id] = std::move(rr);
-      if (r.staged) free_slots(r.slot, r.n);
-    }
-    s.busy = false;
-    cv_done_.notify_all();
-    cv_space_.notify_all();
-  }
+  if (e != hipSuccess) fail_batch(s);
   return true;
+}
+
+// Report a failure to every request of the slot's batch and free the slot.
+void Server::fail_batch(Slot& s) {
+  (void)hipStreamSynchronize(s.ctx->stream());  // no upload may still read the staging ring
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& r : s.reqs) {
+    RequestResult rr;
+    rr.status = -2;
+    rr.n_images = r.n;
+    results_[r.id] = std::move(rr);
+    if (r.staged) free_slots(r.slot, r.n);
+  }
+  s.reqs.clear();
+  s.offsets.clear();
+  s.n_images = 0;
+  s.busy = false;
+  s.open = false;
+  cv_done_.notify_all();
+  cv_space_.notify_all();
+}
+
+bool Server::admit(Slot& s, bool* admitted) {
+  *admitted = false;
+  BatchContext& c = *s.ctx;
+  for (;;) {
+    Request r;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (pending_.empty()) return false;
+      if (s.n_images + pending_.front().n > cfg_.max_batch_images) return true;  // full for the next one
+      r = pending_.front();
+      pending_.pop_front();
+    }
+    s.offsets.push_back(s.n_images);
+    s.reqs.push_back(r);
+    const hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.n_images * slot_bytes_, r.src,
+                                        (uint64_t)r.n * slot_bytes_, hipMemcpyHostToDevice, c.stream());
+    s.n_images += r.n;
+    *admitted = true;
+    if (e != hipSuccess) {
+      fail_batch(s);
+      open_ = -1;
+      return false;
+    }
+    if (s.n_images == cfg_.max_batch_images) return true;
+  }
+}
+
+void Server::launch_open(Slot& s) {
+  BatchContext& c = *s.ctx;
+  hipError_t e = hipSuccess;
+  if (last_computed_) e = hipStreamWaitEvent(c.stream(), last_computed_, 0);
+  if (e == hipSuccess) e = c.run(s.n_images, cfg_.use_graph, /*wait_previous=*/false);
+  if (e == hipSuccess) e = hipEventRecord(s.computed, c.stream());
+  if (e == hipSuccess) last_computed_ = s.computed;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s.h_out, c.out_block(), c.out_bytes(std::min<uint64_t>(kHitPrefix, c.hit_capacity())),
+                       hipMemcpyDeviceToHost, c.stream());
+  if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
+  s.batch_no = batches_.fetch_add(1) + 1;
+  s.open = false;
+  s.busy = true;
+  open_ = -1;
+  if (e != hipSuccess) fail_batch(s);
 }
 
 void Server::harvest(Slot& s) {
@@ -275,6 +329,9 @@ void Server::harvest(Slot& s) {
       results_[s.reqs[i].id] = std::move(rr[i]);
     }
   }
+  s.reqs.clear();
+  s.offsets.clear();
+  s.n_images = 0;
   s.busy = false;
   cv_done_.notify_all();
   cv_space_.notify_all();
@@ -291,21 +348,55 @@ void Server::loop() {
       }
       any_busy = any_busy || s.busy;
     }
-    for (auto& sp : ctx_) {
-      if (sp->busy) continue;
-      if (launch_batch(*sp)) {
-        progress = true;
-        any_busy = true;
+    if (cfg_.in_order_compute) {
+      // open a free slot for arriving requests
+      if (open_ < 0) {
+        bool have;
+        {
+          std::lock_guard<std::mutex> g(mu_);
+          have = !pending_.empty();
+        }
+        for (size_t k = 0; have && k < ctx_.size(); k++)
+          if (!ctx_[k]->busy) {
+            open_ = (int)k;
+            ctx_[k]->open = true;
+            break;
+          }
+      }
+      if (open_ >= 0) {
+        Slot& s = *ctx_[open_];
+        bool admitted = false;
+        const bool full = admit(s, &admitted);
+        progress = progress || admitted;
+        if (open_ >= 0 && s.n_images > 0) {
+          // start the batch when the previous batch's kernels are done (the
+          // GPU would otherwise idle) or when it cannot take the next request
+          const bool idle = !last_computed_ || hipEventQuery(last_computed_) == hipSuccess;
+          if (idle || full) {
+            launch_open(s);
+            progress = true;
+            any_busy = true;
+          }
+        }
+      }
+    } else {
+      for (auto& sp : ctx_) {
+        if (sp->busy) continue;
+        if (launch_batch(*sp)) {
+          progress = true;
+          any_busy = true;
+        }
       }
     }
     if (progress) continue;
+    const bool open_work = open_ >= 0 && ctx_[open_]->n_images > 0;
     std::unique_lock<std::mutex> g(mu_);
-    if (stop_ && pending_.empty() && !any_busy) break;
+    if (stop_ && pending_.empty() && !any_busy && !open_work) break;
     if (!pending_.empty() && !any_busy) continue;  // a context is free: go launch
-    if (any_busy && cfg_.busy_poll) {
+    if ((any_busy || open_work) && cfg_.busy_poll) {
       g.unlock();
       std::this_thread::yield();  // poll the in-flight batches
-    } else if (any_busy)
+    } else if (any_busy || open_work)
       cv_submit_.wait_for(g, std::chrono::microseconds(20));
     else
       cv_submit_.wait_for(g, std::chrono::milliseconds(5), [&] { return stop_ || !pending_.empty(); });
