@@ -97,6 +97,9 @@ class BatchContext {
   hipError_t run(uint32_t n_images, bool use_graph, bool wait_previous = true);
   hipError_t autotune(int iterations, std::vector<TuneRecord>* records);
   MatcherChoice choice_for(uint32_t n_queries) const;
+  // Instantiate now the graph of every matcher choice that some batch size
+  // up to max_images() selects, so that no batch pays for it later.
+  hipError_t prepare_graphs();
 
   hipStream_t stream() const { return stream_; }
   void* input() const { return d_input_; }

@@ -321,6 +321,20 @@ hipError_t BatchContext::graph_for(const MatcherChoice& c, hipGraphExec_t* exec)
   return hipSuccess;
 }
 
+hipError_t BatchContext::prepare_graphs() {
+  std::map<int, MatcherChoice> distinct;
+  for (uint32_t n = 1; n <= max_images_; n++) {
+    const MatcherChoice c = choice_for(n * 8);
+    distinct.emplace(choice_key(c), c);
+  }
+  for (auto& kv : distinct) {
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = graph_for(kv.second, &exec);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t BatchContext::run(uint32_t n_images, bool use_graph, bool wait_previous) {
   if (n_images > max_images_) return hipErrorInvalidValue;
   const MatcherChoice c = choice_for(n_images * 8);

@@ -51,6 +51,8 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
       return e;
     // admission by HBM budget: a context is kept only if it fits
     if (device_bytes() + s->ctx->device_bytes() > cfg.hbm_budget_bytes) break;
+    // every graph a batch size can select is built before the first request
+    if (cfg.use_graph && (e = s->ctx->prepare_graphs()) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s->computed, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out),
