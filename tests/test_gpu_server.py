@@ -70,12 +70,16 @@ def same(a, b):
             and np.array_equal(triples(a["hits"]), triples(b["hits"])))
 
 
-def test_alone_equals_mix(gpu, dev, pool, served_db):
+@pytest.mark.parametrize("in_order", [True, False])
+def test_alone_equals_mix(gpu, dev, pool, served_db, in_order):
+    """Each request's result is the same alone and sharing batches, in both
+    launch policies (in-order compute with upload on admission, and a batch
+    per free context)."""
     import torch
     shards, full = served_db
     d_full = torch.from_numpy(full).to(dev)
     reqs = requests(40, seed=1)
-    srv = gpu.Server(shards, max_batch_images=128, max_contexts=3, threshold=31)
+    srv = gpu.Server(shards, max_batch_images=128, max_contexts=3, threshold=31, in_order_compute=in_order)
     try:
         say("[server] requests alone")
         alone = []

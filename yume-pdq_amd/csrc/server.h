@@ -83,6 +83,7 @@ class Server {
     std::unique_ptr<BatchContext> ctx;
     hipEvent_t done = nullptr;
     hipEvent_t computed = nullptr;  // the batch's kernels have finished
+    hipEvent_t uploaded = nullptr;  // the upload stream has copied the batch's planes
     bool busy = false;     // launched: kernels and readback queued
     bool open = false;     // admitting requests, their uploads queued
     uint64_t batch_no = 0;
@@ -90,8 +91,10 @@ class Server {
     std::vector<Request> reqs;
     std::vector<uint32_t> offsets;  // first image of each request in the batch
     // pinned readback: the context's output block up to the first
-    // kHitPrefix hits (one copy per batch), and the rest of the hits
+    // kHitPrefix hits (written by a kernel at the end of the batch), and the
+    // rest of the hits
     uint8_t* h_out = nullptr;
+    void* d_h_out = nullptr;  // h_out as the GPU addresses it
     MatchHit* h_hits = nullptr;
   };
   void loop();
@@ -101,6 +104,8 @@ class Server {
   bool admit(Slot& s, bool* admitted);
   void launch_open(Slot& s);
   void fail_batch(Slot& s);
+  // upload_ -> context stream dependency, then the batch, then the readback
+  hipError_t enqueue_batch(Slot& s, bool after_last);
   void harvest(Slot& s);
   void free_slots(uint64_t first, uint32_t n);
 
@@ -112,6 +117,7 @@ class Server {
   uint64_t ring_slots_ = 0;
   std::vector<uint8_t> slot_used_;
   uint64_t ring_head_ = 0;  // next slot to hand out
+  hipStream_t upload_ = nullptr;          // every host -> device copy of request planes
   hipEvent_t last_computed_ = nullptr;  // kernels of the last launched batch (in-order compute)
   int open_ = -1;                       // the slot admitting requests (in-order compute)
 

@@ -5,12 +5,19 @@
 // GPU call: whenever a batch context is free it admits queued requests in
 // arrival order until the batch is full, uploads their planes, replays the
 // context's hash -> match graph and records an event; whenever a context's
-// event has fired it reads the batch back and hands each request its hashes,
-// qualities, thresholds and hits.  Contexts run on their own streams, so
-// requests that arrive while batches are on the GPU form the next batch and
-// overlap with them.  The number of contexts -- batches in flight -- is what
-// the HBM budget allows (each holds max_batch_images planes, outputs and a
-// hit buffer), capped at max_contexts.
+// event has fired it hands each request its hashes, qualities, thresholds and
+// hits.  Contexts run on their own streams, so requests that arrive while
+// batches are on the GPU form the next batch and overlap with them.  The
+// number of contexts -- batches in flight -- is what the HBM budget allows
+// (each holds max_batch_images planes, outputs and a hit buffer), capped at
+// max_contexts.
+//
+// Every upload goes on one stream of its own, and a batch's outputs reach the
+// host through a kernel at the end of its graph that stores them into page-
+// locked memory: no DMA copy is queued behind a batch's kernels.  Such a
+// device-to-host copy on the context's stream, enqueued while batches were in
+// flight, blocked the enqueuing thread for 6-10 ms a few times a second, and
+// with it every request behind it.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -25,6 +32,18 @@ namespace {
 // Hits read back with the batch's hashes in one copy; a batch with more takes
 // a second copy for the rest.
 constexpr uint64_t kHitPrefix = 4096;
+
+// Copies a batch's output block into the slot's page-locked readback buffer:
+// the fixed part (count, hashes, qualities, thresholds: fixed16 16-byte
+// words) and the hits up to max_hits (a MatchHit is 16 bytes).
+__global__ void readback_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t fixed16,
+                                const unsigned long long* __restrict__ count, uint64_t max_hits) {
+  const unsigned long long c = *count;
+  const uint64_t n16 = fixed16 + (c < max_hits ? c : max_hits);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+static_assert(sizeof(MatchHit) == 16, "one hit per 16-byte word");
 }  // namespace
 
 Server::~Server() { stop(); }
@@ -55,15 +74,18 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
     if (cfg.use_graph && (e = s->ctx->prepare_graphs()) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s->computed, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&s->uploaded, hipEventDisableTiming)) != hipSuccess) return e;
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out),
                            s->ctx->out_bytes(std::min<uint64_t>(kHitPrefix, cfg.hit_capacity)),
                            hipHostMallocDefault)) ||
+        (e = hipHostGetDevicePointer(&s->d_h_out, s->h_out, 0)) ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_hits), cfg.hit_capacity * sizeof(MatchHit),
                            hipHostMallocDefault)))
       return e;
     ctx_.push_back(std::move(s));
   }
   if (ctx_.empty()) return hipErrorOutOfMemory;  // the budget does not hold one batch
+  if ((e = hipStreamCreateWithFlags(&upload_, hipStreamNonBlocking)) != hipSuccess) return e;
   // room for every context's batch plus as much again queued, so that
   // requests rarely wait for contiguous staging
   ring_slots_ = (uint64_t)(2 * ctx_.size() + 2) * cfg.max_batch_images;
@@ -86,11 +108,16 @@ void Server::stop() {
   for (auto& s : ctx_) {
     if (s->done) (void)hipEventDestroy(s->done);
     if (s->computed) (void)hipEventDestroy(s->computed);
+    if (s->uploaded) (void)hipEventDestroy(s->uploaded);
     void* h[] = {s->h_out, s->h_hits};
     for (void* p : h)
       if (p) (void)hipHostFree(p);
   }
   ctx_.clear();
+  if (upload_) (void)hipStreamDestroy(upload_);
+  upload_ = nullptr;
+  last_computed_ = nullptr;
+  open_ = -1;
   if (staging_) (void)hipHostFree(staging_);
   staging_ = nullptr;
 }
@@ -175,26 +202,43 @@ bool Server::launch_batch(Slot& s) {
     uint64_t bytes = (uint64_t)s.reqs[i].n * slot_bytes_;
     while (j < s.reqs.size() && s.reqs[j].src == s.reqs[i].src + bytes) bytes += (uint64_t)s.reqs[j++].n * slot_bytes_;
     e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.offsets[i] * slot_bytes_, s.reqs[i].src, bytes,
-                       hipMemcpyHostToDevice, c.stream());
+                       hipMemcpyHostToDevice, upload_);
     i = j;
   }
-  // the context's previous batch was harvested, so there is nothing to wait for
-  if (e == hipSuccess) e = c.run(total, cfg_.use_graph, /*wait_previous=*/false);
-  // the outputs and the first hits come back in the same stream order
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(s.h_out, c.out_block(), c.out_bytes(std::min<uint64_t>(kHitPrefix, c.hit_capacity())),
-                       hipMemcpyDeviceToHost, c.stream());
-  if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
   s.n_images = total;
+  if (e == hipSuccess) e = enqueue_batch(s, /*in_order=*/false);
   s.batch_no = batches_.fetch_add(1) + 1;
   s.busy = true;
   if (e != hipSuccess) fail_batch(s);
   return true;
 }
 
+// After the batch's uploads (and, in order, after the last batch's kernels):
+// the hash -> match graph, then the readback kernel and the done event.  The
+// context's previous batch was harvested, so nothing else is waited for.
+hipError_t Server::enqueue_batch(Slot& s, bool in_order) {
+  BatchContext& c = *s.ctx;
+  hipError_t e = hipEventRecord(s.uploaded, upload_);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c.stream(), s.uploaded, 0);
+  if (e == hipSuccess && in_order && last_computed_) e = hipStreamWaitEvent(c.stream(), last_computed_, 0);
+  if (e == hipSuccess) e = c.run(s.n_images, cfg_.use_graph, /*wait_previous=*/false);
+  if (e == hipSuccess) e = hipEventRecord(s.computed, c.stream());
+  if (e == hipSuccess && in_order) last_computed_ = s.computed;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(readback_kernel, dim3(64), dim3(256), 0, c.stream(), static_cast<const uint4*>(c.out_block()),
+                       static_cast<uint4*>(s.d_h_out), c.out_offset(BatchContext::kOutHits) / 16, c.count(),
+                       std::min<uint64_t>(kHitPrefix, c.hit_capacity()));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
+  return e;
+}
+
 // Report a failure to every request of the slot's batch and free the slot.
 void Server::fail_batch(Slot& s) {
-  (void)hipStreamSynchronize(s.ctx->stream());  // no upload may still read the staging ring
+  // no upload may still read the staging ring or the caller's memory
+  (void)hipStreamSynchronize(upload_);
+  (void)hipStreamSynchronize(s.ctx->stream());
   std::lock_guard<std::mutex> g(mu_);
   for (auto& r : s.reqs) {
     RequestResult rr;
@@ -227,7 +271,7 @@ bool Server::admit(Slot& s, bool* admitted) {
     s.offsets.push_back(s.n_images);
     s.reqs.push_back(r);
     const hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(c.input()) + (uint64_t)s.n_images * slot_bytes_, r.src,
-                                        (uint64_t)r.n * slot_bytes_, hipMemcpyHostToDevice, c.stream());
+                                        (uint64_t)r.n * slot_bytes_, hipMemcpyHostToDevice, upload_);
     s.n_images += r.n;
     *admitted = true;
     if (e != hipSuccess) {
@@ -240,16 +284,7 @@ bool Server::admit(Slot& s, bool* admitted) {
 }
 
 void Server::launch_open(Slot& s) {
-  BatchContext& c = *s.ctx;
-  hipError_t e = hipSuccess;
-  if (last_computed_) e = hipStreamWaitEvent(c.stream(), last_computed_, 0);
-  if (e == hipSuccess) e = c.run(s.n_images, cfg_.use_graph, /*wait_previous=*/false);
-  if (e == hipSuccess) e = hipEventRecord(s.computed, c.stream());
-  if (e == hipSuccess) last_computed_ = s.computed;
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(s.h_out, c.out_block(), c.out_bytes(std::min<uint64_t>(kHitPrefix, c.hit_capacity())),
-                       hipMemcpyDeviceToHost, c.stream());
-  if (e == hipSuccess) e = hipEventRecord(s.done, c.stream());
+  const hipError_t e = enqueue_batch(s, /*in_order=*/true);
   s.batch_no = batches_.fetch_add(1) + 1;
   s.open = false;
   s.busy = true;
