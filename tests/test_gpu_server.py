@@ -189,6 +189,29 @@ def test_hit_overflow_rematched(gpu, dev, pool):
         s.close()
 
 
+def test_hits_past_readback_prefix(gpu, dev, pool):
+    """Threshold 256 on a 4000-row shard: 3 images give 24 x 4000 = 96,000
+    hits, more than the 65,536 the batch's readback kernel stores with its
+    hashes; the rest come back by a second copy, and the set is whole."""
+    s = gpu.Shards()
+    s.add(synthetic.random_db(4000, seed=6))
+    srv = gpu.Server(s, max_batch_images=16, threshold=256, hit_capacity=1 << 17)
+    try:
+        say("[server] hits past the readback prefix")
+        r = srv.wait(srv.submit(pool[:3]))
+        assert r["status"] == 0
+        assert r["hits"].size == 24 * 4000
+        q = r["hits"]["query"].astype(np.int64)
+        i = r["hits"]["index"].astype(np.int64)
+        assert np.array_equal(np.sort(q * 4000 + i), np.arange(24 * 4000))
+        # and a batch under the prefix afterwards reads back alone
+        r1 = srv.wait(srv.submit(pool[3:4]))
+        assert r1["status"] == 0 and r1["hits"].size == 8 * 4000
+    finally:
+        srv.close()
+        s.close()
+
+
 def test_submit_validation(gpu, pool, served_db):
     shards, _ = served_db
     srv = gpu.Server(shards, max_batch_images=8)

@@ -248,8 +248,34 @@ struct Outputs {
   float* h_quality = nullptr;
   float* h_threshold = nullptr;
   float* h_pdqf = nullptr;
+  hipEvent_t uploaded = nullptr;  // the batch's frames are in device memory
   hipEvent_t done = nullptr;
 };
+
+// Up to four device -> page-locked host regions, each a whole number of
+// 16-byte words, stored by one kernel: the batch's outputs reach the host
+// without a DMA copy queued behind the hash kernel (such copies blocked the
+// enqueuing thread for milliseconds at a time while work was in flight).
+struct HostStores {
+  const uint4* src[4];
+  uint4* dst[4];
+  uint64_t n16[4];
+};
+
+__global__ void store_to_host_kernel(HostStores r) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r.n16[k]; i += step) r.dst[k][i] = r.src[k][i];
+}
+
+template <typename T>
+hipError_t device_view(T* host, T** dev) {
+  void* p = nullptr;
+  const hipError_t e = hipHostGetDevicePointer(&p, host, 0);
+  *dev = static_cast<T*>(p);
+  return e;
+}
 
 }  // namespace
 
@@ -263,8 +289,11 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
   Slot slots[kSlots];
   Outputs outs[2];
   void* d_in[2] = {nullptr, nullptr};
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, upload = nullptr;  // hashing and readback; frame uploads
   hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&upload, hipStreamNonBlocking);
+  HostStores stores[2];
+  memset(stores, 0, sizeof stores);
   auto fail_hip = [&](hipError_t err) {
     if (error) *error = std::string("HIP: ") + hipGetErrorString(err);
     return -2;
@@ -283,6 +312,16 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&x.h_threshold, (size_t)B * 32, 0);
     if (e == hipSuccess && diag) e = hipHostMalloc((void**)&x.h_pdqf, (size_t)B * 8 * 256 * 4, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&x.uploaded, hipEventDisableTiming);
+    HostStores& r = stores[k];
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_hashes), &r.dst[0]);
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_quality), &r.dst[1]);
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_threshold), &r.dst[2]);
+    if (e == hipSuccess && diag) e = device_view(reinterpret_cast<uint4*>(x.h_pdqf), &r.dst[3]);
+    r.src[0] = reinterpret_cast<const uint4*>(x.d_hashes);
+    r.src[1] = reinterpret_cast<const uint4*>(x.d_quality);
+    r.src[2] = reinterpret_cast<const uint4*>(x.d_threshold);
+    r.src[3] = reinterpret_cast<const uint4*>(x.d_pdqf);
   }
 
   std::mutex mu;
@@ -393,16 +432,21 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
       ho.buf1 = nullptr;
       ho.tmp = nullptr;
       ho.dihedral_pdqf = diag ? x.d_pdqf : nullptr;
-      hipError_t err = hipMemcpyAsync(d_in[oi], s.frames, s.n * frame_bytes, hipMemcpyHostToDevice, stream);
+      // d_in[oi] and outs[oi] were last used two batches ago, whose done
+      // event the host has already waited for
+      hipError_t err = hipMemcpyAsync(d_in[oi], s.frames, s.n * frame_bytes, hipMemcpyHostToDevice, upload);
+      if (err == hipSuccess) err = hipEventRecord(x.uploaded, upload);
+      if (err == hipSuccess) err = hipStreamWaitEvent(stream, x.uploaded, 0);
       if (err == hipSuccess) err = launch_hash(d_in[oi], o.input_kind, frame_bytes, s.n, o.dihedral_mode, ho, stream);
-      if (err == hipSuccess)
-        err = hipMemcpyAsync(x.h_hashes, x.d_hashes, (size_t)s.n * 256, hipMemcpyDeviceToHost, stream);
-      if (err == hipSuccess)
-        err = hipMemcpyAsync(x.h_quality, x.d_quality, (size_t)s.n * 32, hipMemcpyDeviceToHost, stream);
-      if (err == hipSuccess)
-        err = hipMemcpyAsync(x.h_threshold, x.d_threshold, (size_t)s.n * 32, hipMemcpyDeviceToHost, stream);
-      if (err == hipSuccess && diag)
-        err = hipMemcpyAsync(x.h_pdqf, x.d_pdqf, (size_t)s.n * 8 * 256 * 4, hipMemcpyDeviceToHost, stream);
+      if (err == hipSuccess) {
+        HostStores r = stores[oi];
+        r.n16[0] = (uint64_t)s.n * 256 / 16;
+        r.n16[1] = (uint64_t)s.n * 32 / 16;
+        r.n16[2] = (uint64_t)s.n * 32 / 16;
+        r.n16[3] = diag ? (uint64_t)s.n * 8 * 256 * 4 / 16 : 0;
+        hipLaunchKernelGGL(store_to_host_kernel, dim3(diag ? 256 : 32), dim3(256), 0, stream, r);
+        err = hipGetLastError();
+      }
       if (err == hipSuccess) err = hipEventRecord(x.done, stream);
       if (err != hipSuccess) {
         rc = fail_hip(err);
@@ -436,6 +480,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
   cv.notify_all();
   if (reader.joinable()) reader.join();
   (void)hipStreamSynchronize(stream);
+  if (upload) (void)hipStreamSynchronize(upload);
   for (int s = 0; s < kSlots; s++)
     if (slots[s].frames) (void)hipHostFree(slots[s].frames);
   for (int k = 0; k < 2; k++) {
@@ -450,8 +495,10 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     if (x.h_threshold) (void)hipHostFree(x.h_threshold);
     if (x.h_pdqf) (void)hipHostFree(x.h_pdqf);
     if (x.done) (void)hipEventDestroy(x.done);
+    if (x.uploaded) (void)hipEventDestroy(x.uploaded);
   }
   if (stream) (void)hipStreamDestroy(stream);
+  if (upload) (void)hipStreamDestroy(upload);
   if (frames_out) *frames_out = frames;
   return rc;
 }

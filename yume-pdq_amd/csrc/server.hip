@@ -29,9 +29,12 @@
 namespace ypdq {
 
 namespace {
-// Hits read back with the batch's hashes in one copy; a batch with more takes
-// a second copy for the rest.
-constexpr uint64_t kHitPrefix = 4096;
+// Hits stored to the host with the batch's hashes by the readback kernel,
+// which writes only the hits there are (1 MiB of page-locked memory per
+// context); a batch with more takes a second, synchronous copy for the rest.
+// Near-duplicate traffic reaches tens of hits per image, so a few thousand
+// would not cover a full batch.
+constexpr uint64_t kHitPrefix = 65536;
 
 // Copies a batch's output block into the slot's page-locked readback buffer:
 // the fixed part (count, hashes, qualities, thresholds: fixed16 16-byte
