@@ -117,17 +117,27 @@ def main():
     d_count = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
 
+    hm_cache = {}
+
+    def hash_match_for(planes, B, plane):
+        key = (planes.data_ptr(), planes.dtype, B, plane is not None)
+        if key not in hm_cache:
+            hm_cache[key] = yume_pdq.HashMatch(planes[:B], d_db, thr, plane=plane, capacity=1 << 20, stream=stream)
+        return hm_cache[key]
+
     def step(planes, B, ev=None, plane=d_plane):
+        """One batch through the one-call path (yume_pdq_amd_hash_match_batch:
+        hash kernel, counter reset, matcher back to back, the hit count stored
+        into page-locked host memory); the step ends when the count is on the
+        host."""
+        hm = hash_match_for(planes, B, plane)
         if ev is not None:
             ev[0].record(stream)
-        h, _, _ = yume_pdq.hash_batch(planes[:B], stream=stream, out=out_bufs[B])
+        hm.run(sync=False)
         if ev is not None:
             ev[1].record(stream)
-        yume_pdq.match(d_db, h.reshape(-1, 32), thr, hits=d_hits, count=d_count, sync=False,
-                       stream=stream, plane=plane)
-        if ev is not None:
-            ev[2].record(stream)
-        return int(d_count.item())  # the step ends when its hit count is on the host
+        stream.synchronize()
+        return int(hm.h_count[0])
 
     out_bufs = {}
     for B in set(batches + [args.headline_batch, 1]):
@@ -138,7 +148,7 @@ def main():
     def run(planes, B, plane=d_plane):
         for _ in range(args.warmup):
             step(planes, B, plane=plane)
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
         lat = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -149,19 +159,53 @@ def main():
             lat.append((time.perf_counter() - s0) * 1e3)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        t_hash = [e[0].elapsed_time(e[1]) for e in evs]
-        t_match = [e[1].elapsed_time(e[2]) for e in evs]
+        t_gpu = [e[0].elapsed_time(e[1]) for e in evs]
         total_s = t1 - t0
+        # the step's two kernels timed apart, through the separate entry points
+        queries = hash_match_for(planes, B, plane).hashes.reshape(-1, 32)
+        t_hash = kernel_ms(lambda: yume_pdq.hash_batch(planes[:B], stream=stream, out=out_bufs[B]), k=10)
+        t_match = kernel_ms(lambda: match_appending(queries, plane), k=10)
         return {
             "batch_images": B,
             "queries_per_pass": 8 * B,
             "images_per_s": B * args.steps / total_s,
             "ms_per_step": total_s * 1e3 / args.steps,
             "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-            "hash_kernel_ms": float(np.mean(t_hash)),
-            "match_kernel_ms": float(np.mean(t_match)),
+            "step_gpu_ms": float(np.mean(t_gpu)),
+            "hash_kernel_ms": t_hash,
+            "match_kernel_ms": t_match,
             "hits": hits,
         }
+
+    # Kernel durations (kernel_ms).  The match calls append to the hit
+    # counter, so no reset kernel runs between the timed launches.
+    def kernel_ms(fn, k=20):
+        """Average GPU duration of one launch of fn on `stream`: k launches
+        back to back between one pair of events.  About 20 ms of full-size
+        matching is queued first (40 launches at 1024 queries, a few ms of
+        host time), so the host has queued every timed launch before the GPU
+        reaches them: they run back to back at the sustained clock, as in a
+        kernel trace, and no host-side gap (tens of us of Python and ctypes
+        per call, longer than a 25 us scan) falls between the events."""
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+        r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(40):
+            match_appending(q_head, d_plane)
+        r0.record(stream)
+        for _ in range(k):
+            fn()
+        r1.record(stream)
+        r1.synchronize()
+        return r0.elapsed_time(r1) / k
+
+    q_head = out_bufs[args.headline_batch][0].reshape(-1, 32)
+    q_one = out_bufs[1][0].reshape(-1, 32)
+
+    def match_appending(q, plane):
+        yume_pdq.match(d_db, q, thr, hits=d_hits, count=d_count, sync=False, stream=stream, plane=plane,
+                       append=True)
 
     # The GPU raises its clock over the first tens of milliseconds of
     # sustained load; untimed steps first, so the timed ones see the
@@ -192,38 +236,7 @@ def main():
     n_rows = args.db_rows
     B = args.headline_batch
 
-    # Kernel durations: K launches, each between its own pair of HIP events
-    # on the launch stream (so host-side gaps between calls are not counted),
-    # mean per launch.  The match calls append to the hit counter, so each
-    # bracket holds the matcher's launch alone.
-    def kernel_ms(fn, k=20):
-        """Average GPU duration of one launch of fn on `stream`: k launches
-        back to back between one pair of events.  About 20 ms of full-size
-        matching is queued first (40 launches at 1024 queries, a few ms of
-        host time), so the host has queued every timed launch before the GPU
-        reaches them: they run back to back at the sustained clock, as in a
-        kernel trace, and no host-side gap (tens of us of Python and ctypes
-        per call, longer than a 25 us scan) falls between the events."""
-        for _ in range(10):
-            fn()
-        torch.cuda.synchronize()
-        r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        for _ in range(40):
-            match_appending(q_head, d_plane)
-        r0.record(stream)
-        for _ in range(k):
-            fn()
-        r1.record(stream)
-        r1.synchronize()
-        return r0.elapsed_time(r1) / k
-
-    q_head = out_bufs[B][0].reshape(-1, 32)
-    q_one = out_bufs[1][0].reshape(-1, 32)
     log("[bench] back-to-back kernel timing")
-    def match_appending(q, plane):
-        yume_pdq.match(d_db, q, thr, hits=d_hits, count=d_count, sync=False, stream=stream, plane=plane,
-                       append=True)
-
     d_count.zero_()
     yume_pdq.match(d_db, q_head, thr, hits=d_hits, count=d_count, sync=False, stream=stream, plane=d_plane)
     hits_head = int(d_count.item())
@@ -344,13 +357,16 @@ def main():
             for off in [float(x) for x in args.serve_offered_100m.split(",")]
         ]
         serving.extend(runs100)
-        top100 = max(runs100, key=lambda r: r["served_images_per_s"])
+        # the highest rate the server kept up with (served >= 95 % of offered)
+        kept = [r for r in runs100 if r["served_images_per_s"] >= 0.95 * r["offered_images_per_s"]]
+        top100 = max(kept or runs100, key=lambda r: r["served_images_per_s"])
         serving_summary["100m_rows_best_served"] = {
             "offered_images_per_s": top100["offered_images_per_s"],
             "served_images_per_s": top100["served_images_per_s"], "latency_ms": top100["latency_ms"],
             "served_vs_plain": top100["served_vs_plain"],
-            "note": ("the 100M-row database of the serving configuration; one 64-image request there needs "
-                     "about 2.4 ms of matching, so its p99 stays above 2 ms")}
+            "note": ("the highest rate served at >= 95 % of offered on the 100M-row database of the serving "
+                     "configuration; one 64-image request there needs about 2.4 ms of matching, so its p99 stays "
+                     "above 2 ms")}
         del d_db100
         torch.cuda.empty_cache()
     uneven = None

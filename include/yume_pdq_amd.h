@@ -211,6 +211,28 @@ int yume_pdq_amd_match_with_plane(const uint8_t *d_db, const void *d_plane, uint
                                   YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
                                   uint64_t index_base, uint32_t flags, void *stream);
 
+/*
+ * Hash and match in one call: yume_pdq_amd_hash_batch of n_images planes, then
+ * the match of their 8 n_images dihedral hashes (query 8 i + d: image i,
+ * orientation d) against the database -- over d_plane when given (as
+ * yume_pdq_amd_match_with_plane), else over the raw rows -- with the hit
+ * counter reset first, all enqueued back to back on `stream` after every
+ * argument has been checked.  When h_count is not NULL it must be page-locked
+ * host memory from hipHostMalloc: the GPU stores the hit count there after the
+ * matcher, so once `stream` is synchronised the caller reads it without a
+ * copy.  The call does not wait.
+ * No reference counterpart: the batched hash + match of the new device ABI
+ * (one host call per batch where the reference hashes per image,
+ * src/lib.rs:251-283, and matches per 2048-row batch,
+ * src/matching/mod.rs:232-257).
+ */
+int yume_pdq_amd_hash_match_batch(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
+                                  uint32_t n_images, uint32_t dihedral_mode, const uint8_t *d_db,
+                                  const void *d_plane, uint64_t n_rows, uint32_t threshold,
+                                  uint8_t *d_hashes, float *d_quality, float *d_threshold,
+                                  YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
+                                  unsigned long long *h_count, void *stream);
+
 /* Which kernel yume_pdq_amd_match would run for these arguments:
  * YUME_PDQ_AMD_MATCH_VALU or YUME_PDQ_AMD_MATCH_MFMA (or EINVAL for
  * contradictory flags).  No reference counterpart (path introspection for

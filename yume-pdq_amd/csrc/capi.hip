@@ -131,6 +131,8 @@ int from_hip(hipError_t e) {
 }
 
 __global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+// the hit count into page-locked host memory (its device address)
+__global__ void store_count(const unsigned long long* d, unsigned long long* h) { *h = *d; }
 
 }  // namespace
 
@@ -379,6 +381,43 @@ int yume_pdq_amd_match_with_plane(const uint8_t* d_db, const void* d_plane, uint
   if (!d_plane && n_rows) return YUME_PDQ_AMD_EINVAL;
   return match_impl(d_db, d_plane, n_rows, d_queries, n_queries, threshold, d_hits, capacity, d_count, index_base,
                     flags, stream);
+}
+
+int yume_pdq_amd_hash_match_batch(const void* d_planes, int input_kind, uint64_t plane_stride_bytes,
+                                  uint32_t n_images, uint32_t dihedral_mode, const uint8_t* d_db,
+                                  const void* d_plane, uint64_t n_rows, uint32_t threshold, uint8_t* d_hashes,
+                                  float* d_quality, float* d_threshold, YumePdqAmdHit* d_hits, uint64_t capacity,
+                                  unsigned long long* d_count, unsigned long long* h_count, void* stream) {
+  // every argument first, so that an error enqueues nothing
+  if (n_images > UINT32_MAX / 8) return YUME_PDQ_AMD_EINVAL;
+  if (n_images && (!d_planes || !d_hashes || !d_quality || !d_threshold)) return YUME_PDQ_AMD_EINVAL;
+  const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
+  if (plane_bytes == 0 || dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
+  if (n_images && (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16))
+    return YUME_PDQ_AMD_EINVAL;
+  if (!d_count || (capacity && !d_hits) || (n_rows && !d_db)) return YUME_PDQ_AMD_EINVAL;
+  if ((uintptr_t)d_db % 16 || (uintptr_t)d_plane % 16) return YUME_PDQ_AMD_EINVAL;
+  unsigned long long* d_h_count = nullptr;
+  if (h_count) {
+    // page-locked host memory from hipHostMalloc only: the GPU stores into it
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, h_count) != hipSuccess || a.type != hipMemoryTypeHost ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&d_h_count), h_count, 0) != hipSuccess || !d_h_count) {
+      (void)hipGetLastError();  // the failed query must not surface in a later call
+      return YUME_PDQ_AMD_EINVAL;
+    }
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = yume_pdq_amd_hash_batch(d_planes, input_kind, plane_stride_bytes, n_images, dihedral_mode, d_hashes,
+                                   d_quality, d_threshold, nullptr, stream);
+  if (rc == YUME_PDQ_AMD_OK)
+    rc = match_impl(d_db, d_plane, n_rows, d_hashes, 8 * n_images, threshold, d_hits, capacity, d_count, 0, 0,
+                    stream);
+  if (rc == YUME_PDQ_AMD_OK && d_h_count) {
+    hipLaunchKernelGGL(store_count, dim3(1), dim3(1), 0, st, d_count, d_h_count);
+    rc = from_hip(hipGetLastError());
+  }
+  return rc;
 }
 
 uint64_t yume_pdq_amd_prefix_plane_bytes(uint64_t n_rows) { return ypdq::prefix_plane_bytes(n_rows); }

@@ -126,6 +126,9 @@ def lib():
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
         L.yume_pdq_amd_match_with_plane.restype = i
         L.yume_pdq_amd_match_with_plane.argtypes = [vp, vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
+        L.yume_pdq_amd_hash_match_batch.restype = i
+        L.yume_pdq_amd_hash_match_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, u64, u32, vp, vp, vp, vp, u64,
+                                                    vp, vp, vp]
         L.yume_pdq_amd_prefix_plane_bytes.restype = u64
         L.yume_pdq_amd_prefix_plane_bytes.argtypes = [u64]
         L.yume_pdq_amd_build_prefix_plane.restype = i
@@ -404,6 +407,77 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
     arr = hits[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
     arr = np.sort(arr, order=("index", "query"))
     return total, arr
+
+
+class HashMatch:
+    """Hash and match one batch shape repeatedly with one native call per batch
+    (yume_pdq_amd_hash_match_batch): n u8 or f32 planes (n, 512, 512) resident
+    on the GPU, their 8n dihedral hashes matched against db (n_rows, 32) --
+    over its prefix plane when given -- at `threshold`.  Every buffer is
+    allocated here once; run() enqueues the batch on `stream` and, unless
+    sync is False, waits for it and returns the hit count, which the GPU
+    stores into page-locked host memory.  Outputs: .hashes (n, 8, 32),
+    .quality / .threshold (n, 8), .hits (capacity, 16) u8 records of
+    HIT_DTYPE, .count (1,) int64 on the device."""
+
+    def __init__(self, planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_EXACT,
+                 stream=None):
+        torch = _torch()
+        if not planes.is_cuda or planes.dim() != 3 or tuple(planes.shape[1:]) != (512, 512):
+            raise ValueError("planes must be a CUDA tensor of shape (n, 512, 512)")
+        if planes.dtype == torch.uint8:
+            kind, elem = INPUT_U8, 1
+        elif planes.dtype == torch.float32:
+            kind, elem = INPUT_F32, 4
+        else:
+            raise ValueError("planes must be uint8 or float32")
+        self.planes = planes.contiguous()
+        self.db = db.reshape(-1, 32).contiguous()
+        if not self.db.is_cuda or self.db.dtype != torch.uint8:
+            raise ValueError("db must be a CUDA uint8 tensor of 32-byte rows")
+        self.plane = plane
+        n, dev = self.planes.shape[0], self.planes.device
+        self.hashes = torch.empty((n, 8, 32), dtype=torch.uint8, device=dev)
+        self.quality = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        self.threshold = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        self.hits = torch.empty((max(capacity, 1), 16), dtype=torch.uint8, device=dev)
+        self.count = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.h_count = torch.zeros(1, dtype=torch.int64).pin_memory()
+        self.stream = stream if stream is not None else torch.cuda.current_stream(dev)
+        self._args = (C.c_void_p(self.planes.data_ptr()), kind, 512 * 512 * elem, n, int(dihedral_mode),
+                      C.c_void_p(self.db.data_ptr()), C.c_void_p(plane.data_ptr() if plane is not None else 0),
+                      self.db.shape[0], int(threshold), C.c_void_p(self.hashes.data_ptr()),
+                      C.c_void_p(self.quality.data_ptr()), C.c_void_p(self.threshold.data_ptr()),
+                      C.c_void_p(self.hits.data_ptr()), self.hits.shape[0], C.c_void_p(self.count.data_ptr()),
+                      C.c_void_p(self.h_count.data_ptr()), C.c_void_p(self.stream.cuda_stream))
+        self._fn = lib().yume_pdq_amd_hash_match_batch
+
+    def run(self, sync=True):
+        rc = self._fn(*self._args)
+        if rc < 0:
+            _check(rc, "hash_match_batch")
+        if not sync:
+            return None
+        self.stream.synchronize()
+        return int(self.h_count[0])
+
+    def results(self):
+        """(total, hits sorted by (index, query)) of the last run, after it
+        has finished."""
+        total = int(self.h_count[0])
+        k = min(total, self.hits.shape[0])
+        arr = self.hits[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
+        return total, np.sort(arr, order=("index", "query"))
+
+
+def hash_match(planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_EXACT, stream=None):
+    """One batch through yume_pdq_amd_hash_match_batch: returns (hashes (n,8,32),
+    quality (n,8), threshold (n,8), total, hits sorted by (index, query))."""
+    hm = HashMatch(planes, db, threshold, plane=plane, capacity=capacity, dihedral_mode=dihedral_mode,
+                   stream=stream)
+    hm.run()
+    total, arr = hm.results()
+    return hm.hashes, hm.quality, hm.threshold, total, arr
 
 
 def match_uses_matrix_cores(n_queries, threshold, path=None):
