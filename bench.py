@@ -5,7 +5,9 @@ One step = hash B synthetic 512x512 u8 images resident in HBM (fused kernel:
 tent compression, DCT, quality, median quantisation, 8 dihedral hashes), then
 match the 8B dihedral hashes against a 10M-row database resident in HBM at
 threshold 31 (exact (query, index, distance) triples into a device buffer),
-then read the hit count back.  The database carries 50,000 planted
+then read the hit count back -- one library call per step
+(yume_pdq_amd_hash_match_batch), the count stored into page-locked host
+memory by the GPU.  The database carries 50,000 planted
 near-duplicates (U{0..40} distinct bit flips) of the 8192 dihedral hashes of
 1024 images; a batch of B images is the first B of them.
 
