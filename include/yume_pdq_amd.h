@@ -377,12 +377,17 @@ int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher *b, uint32_t n_queries, 
  * Serving loop (continuous batching).  No reference counterpart.  Requests of
  * 1..max_batch_images u8 planes are copied into pinned staging by submit()
  * and queued; a worker thread admits queued requests in arrival order into
- * the next free batch context (as many contexts as hbm_budget_bytes holds,
- * at most max_contexts, so batches overlap on the device), replays its
- * hash -> match graph, and hands each request its own hashes, qualities,
- * thresholds and hits.  A request's results do not depend on which other
- * requests shared its batch.  Hit queries are 8 * (image in the request) +
- * dihedral; indices are global.
+ * a free batch context (as many contexts as hbm_budget_bytes holds, at most
+ * max_contexts), uploading each request's planes as it is admitted, replays
+ * the context's hash -> match graph, and hands each request its own hashes,
+ * qualities, thresholds and hits.  With in_order_compute (the default) one
+ * context admits at a time and its batch starts when the previous batch's
+ * kernels have finished or when the next request would not fit, so uploads
+ * overlap the batches in flight; otherwise every free context takes what is
+ * queued at once.  A batch's outputs come back through page-locked memory
+ * written by a kernel at its end.  A request's results do not depend on which
+ * other requests shared its batch.  Hit queries are 8 * (image in the
+ * request) + dihedral; indices are global.
  */
 typedef struct YumePdqAmdServer YumePdqAmdServer;
 typedef struct YumePdqAmdServerConfig {
