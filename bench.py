@@ -837,6 +837,10 @@ def cpu_baseline(args, db, thr, n_rows):
     hits, _, t_hash, t_match = oracle_lib.cpu_baseline(pool, db, thr, threads, use_avx512=use512,
                                                        n_images=n_img)
     total = t_hash + t_match
+    # one thread, as the reference's own single-core figures (README.md:191-195, bench.csv)
+    n1 = 64
+    log(f"[bench] CPU baseline, one thread: {n1} images, {8 * n1} queries x {n_rows} rows")
+    _, _, t1_hash, t1_match = oracle_lib.cpu_baseline(pool, db, thr, 1, use_avx512=use512, n_images=n1)
     return {
         "value": round(n_img / total, 2),
         "unit": "images/s",
@@ -849,7 +853,12 @@ def cpu_baseline(args, db, thr, n_rows):
                    f"{threads} threads"),
         "hash_s": round(t_hash, 4),
         "match_s": round(t_match, 4),
+        "hash_images_per_s": round(n_img / t_hash, 1),
+        "match_images_per_s": round(n_img / t_match, 1),
         "hits": hits,
+        "one_thread": {"images": n1, "hash_images_per_s": round(n1 / t1_hash, 1),
+                       "match_images_per_s": round(n1 / t1_match, 1),
+                       "hash_match_images_per_s": round(n1 / (t1_hash + t1_match), 2)},
     }
 
 
