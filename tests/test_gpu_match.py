@@ -89,11 +89,14 @@ def test_thresholds(gpu, oracle, dev, threshold, path):
 
 
 @pytest.mark.parametrize("plane", [False, True])
-@pytest.mark.parametrize("nq", [1, 15, 16, 17, 33, 100, 257, 1000])
+@pytest.mark.parametrize("nq", [1, 15, 16, 17, 33, 100, 257, 385, 513, 1000, 2048])
 def test_mfma_query_counts(gpu, oracle, dev, nq, plane):
     """Ragged query counts on the matrix-core path (partial 16-query tiles,
     partial query groups) against the oracle, ragged row counts too (5003
-    rows: a partial last tile and a partial last wave), raw rows and plane."""
+    rows: a partial last tile and a partial last wave), raw rows and plane.
+    From 4 query groups the block's waves share their rows (385: a last wave
+    with one query; 513: a second group of waves with one active; 1000, 2048:
+    full and partial groups of waves)."""
     import torch
     db = synthetic.random_db(5003, seed=40 + nq)
     q = synthetic.random_db(nq, seed=41)
@@ -104,6 +107,80 @@ def test_mfma_query_counts(gpu, oracle, dev, nq, plane):
                            "mfma_plane" if plane else "mfma")
     assert total == exp_n
     assert np.array_equal(sort_triples(as_triples(hits)), exp)
+
+
+@pytest.mark.parametrize("plane", [False, True])
+def test_mfma_dense_candidates(gpu, oracle, dev, plane):
+    """Threshold 63 puts the prefix bound at 2, which about half of all
+    random pairs reach: every group of tiles of every wave is a candidate and
+    goes to the second pass, pair by pair.  150,003 rows x 2048 queries (147
+    shared sub-chunks x 16 query groups, a partial last sub-chunk) against the
+    oracle."""
+    import torch
+    db = synthetic.random_db(150_003, seed=90)
+    q = synthetic.random_db(2048, seed=91)
+    synthetic.plant(db, q, 2000, seed=92, max_dist=70)
+    say(f"[match] mfma dense candidates plane={plane}")
+    exp_n, exp = oracle.match_all(db, q, 63)
+    total, hits = run_path(gpu, torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), 63,
+                           "mfma_plane" if plane else "mfma")
+    assert total == exp_n
+    assert np.array_equal(sort_triples(as_triples(hits)), exp)
+
+
+def near(base, n, max_flips, seed):
+    """n copies of the 32-byte hash `base`, each with up to max_flips bits flipped."""
+    rng = np.random.default_rng(seed)
+    out = np.repeat(base[None, :], n, axis=0)
+    for i in range(n):
+        for b in rng.choice(256, size=int(rng.integers(0, max_flips + 1)), replace=False):
+            out[i, b // 8] ^= np.uint8(1 << (b % 8))
+    return out
+
+
+@pytest.mark.parametrize("plane", [False, True])
+def test_mfma_every_pair_a_hit(gpu, oracle, dev, plane):
+    """2048 queries and 300 rows, all within 8 bits of one hash: every pair is
+    a hit at threshold 31 (614,400 hits from 16 candidate records, far more
+    than a block of the second pass gathers before it appends directly).  The
+    set equals the oracle's; with a 1000-hit buffer the count stays exact and
+    the stored hits are 1000 distinct pairs of the set."""
+    import torch
+    base = synthetic.random_db(1, seed=96)[0]
+    q = near(base, 2048, 8, seed=97)
+    db = near(base, 300, 8, seed=98)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
+    say(f"[match] every pair a hit plane={plane}")
+    exp_n, exp = oracle.match_all(db, q, 31, cap=300 * 2048)
+    assert exp_n == 300 * 2048
+    total, hits = run_path(gpu, d_db, d_q, 31, "mfma_plane" if plane else "mfma", capacity=1 << 20)
+    assert total == exp_n
+    assert np.array_equal(sort_triples(as_triples(hits)), exp)
+    say(f"[match] every pair a hit, 1000-hit buffer plane={plane}")
+    total, hits = run_path(gpu, d_db, d_q, 31, "mfma_plane" if plane else "mfma", capacity=1000)
+    assert total == exp_n and hits.size == 1000
+    t = as_triples(hits)
+    assert len(set(map(tuple, t.tolist()))) == 1000
+    assert set(map(tuple, t.tolist())) <= set(map(tuple, exp.tolist()))
+
+
+def test_mfma_candidate_records_overflow(gpu, dev):
+    """5M rows x 2048 queries at threshold 63: 4883 sub-chunks x 16 query
+    groups, every one a candidate, are more candidate records than the 65,536
+    the first pass keeps, so the second pass matches every work item again
+    with per-wave candidate checks.  The set equals the vector-ALU kernel's,
+    and holds the planted pairs."""
+    import torch
+    db = synthetic.random_db(5_000_000, seed=93)
+    q = synthetic.random_db(2048, seed=94)
+    synthetic.plant_fast(db, q, 20_000, seed=95)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
+    say("[match] record overflow: mfma")
+    n1, h1 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="mfma")
+    say("[match] record overflow: valu")
+    n2, h2 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="valu")
+    assert n1 == n2 and np.array_equal(as_triples(h1), as_triples(h2))
+    assert n1 >= 0.9 * 20_000
 
 
 def test_prefix_plane_layout(gpu, dev):

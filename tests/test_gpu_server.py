@@ -74,12 +74,15 @@ def same(a, b):
 def test_alone_equals_mix(gpu, dev, pool, served_db, in_order):
     """Each request's result is the same alone and sharing batches, in both
     launch policies (in-order compute with upload on admission, and a batch
-    per free context)."""
+    per free context).  The second policy starts a batch on any free context
+    at once, so it runs with one context: requests then queue behind the
+    batch in flight and share the next one."""
     import torch
     shards, full = served_db
     d_full = torch.from_numpy(full).to(dev)
     reqs = requests(40, seed=1)
-    srv = gpu.Server(shards, max_batch_images=128, max_contexts=3, threshold=31, in_order_compute=in_order)
+    srv = gpu.Server(shards, max_batch_images=128, max_contexts=3 if in_order else 1, threshold=31,
+                     in_order_compute=in_order)
     try:
         say("[server] requests alone")
         alone = []

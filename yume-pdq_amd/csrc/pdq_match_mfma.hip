@@ -21,12 +21,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "pdq_device.h"
 
 namespace ypdq {
 
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaWaves = 4;
 constexpr int kMfmaTilesPerWave = 64;  // 16-row tiles streamed per wave per block
@@ -67,11 +73,11 @@ __device__ __forceinline__ uint32_t full_distance(const uint32_t* __restrict__ a
 }
 
 // Exact check of query tile t's candidates in this lane's four outputs.
-__device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
-                                                float bound, const uint32_t* __restrict__ queries,
-                                                const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
-                                                uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
-                                                unsigned long long* __restrict__ counter, uint64_t index_base) {
+__device__ __forceinline__ void check_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
+                                                 float bound, const uint32_t* __restrict__ queries,
+                                                 const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
+                                                 uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
+                                                 unsigned long long* __restrict__ counter, uint64_t index_base) {
   const uint64_t row = tbase + rn;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
@@ -88,6 +94,59 @@ __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, u
             h.distance = d;
             h.index = index_base + row;
             hits[slot] = h;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The same out of line, for the per-wave layout, whose rare path keeps the
+// stream's query operands (inlined there it spills registers into the tile
+// loop), and for the shared-rows layout's second kernel.  (The shared-rows
+// streaming kernel checks nothing itself: a call anywhere in it, even one
+// that never runs, cost its tile loop about 12 %, and inline checks 9 %.)
+__device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
+                                                float bound, const uint32_t* __restrict__ queries,
+                                                const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
+                                                uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
+                                                unsigned long long* __restrict__ counter, uint64_t index_base) {
+  check_candidates(cv, t, tbase, qbase, rn, kq, bound, queries, db, n_queries, n_rows, threshold, hits, capacity,
+                   counter, index_base);
+}
+
+// The shared-rows layout's second kernel appends to its block's hit list in
+// LDS (stage, n_stage) and takes its slots in the hit buffer once per block:
+// thousands of appends to the one global counter from a short kernel
+// serialise on it (measured: 49 of 68 microseconds for 4938 hits).  Hits past
+// the list's kStageHits go straight to the buffer.
+constexpr uint32_t kStageHits = 512;
+
+__device__ __noinline__ void stage_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
+                                              float bound, const uint32_t* __restrict__ queries,
+                                              const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
+                                              uint32_t threshold, MatchHit* stage, uint32_t* n_stage,
+                                              MatchHit* __restrict__ hits, uint64_t capacity,
+                                              unsigned long long* __restrict__ counter, uint64_t index_base) {
+  const uint64_t row = tbase + rn;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float v = r == 0 ? cv.x : r == 1 ? cv.y : r == 2 ? cv.z : cv.w;
+    if (v >= bound) {
+      const uint32_t qi = qbase + 16 * t + 4 * kq + r;
+      if (qi < n_queries && row < n_rows) {
+        const uint32_t d = full_distance(queries + (size_t)qi * 8, db + row * 8);
+        if (d <= threshold) {
+          MatchHit h;
+          h.query = qi;
+          h.distance = d;
+          h.index = index_base + row;
+          const uint32_t s = atomicAdd(n_stage, 1u);
+          if (s < kStageHits) {
+            stage[s] = h;
+          } else {
+            const unsigned long long slot = atomicAdd(counter, 1ull);
+            if (slot < capacity) hits[slot] = h;
           }
         }
       }
@@ -217,16 +276,16 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
 }
 
 // One work item: row chunk `chunk` of a database (kMfmaWaves waves x
-// kMfmaTilesPerWave tiles x 16 rows; each wave streams its own rows) against
-// query group `group` (QT x 16 queries).
+// kMfmaTilesPerWave tiles x 16 rows; each wave, `wave` = 0 .. kMfmaWaves - 1,
+// streams its own rows) against query group `group` (QT x 16 queries).
 template <int QT, bool kPlane>
 __device__ __forceinline__ void mfma_item(const uint32_t* __restrict__ lut, const uint32_t* __restrict__ db,
                                           const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t chunk,
                                           uint32_t group, const uint32_t* __restrict__ queries, uint32_t n_queries,
                                           uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity,
-                                          unsigned long long* __restrict__ counter, uint64_t index_base) {
+                                          unsigned long long* __restrict__ counter, uint64_t index_base,
+                                          int wave) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int rn = lane & 15;  // operand row (query m / db row n) of this lane
   const int kq = lane >> 4;  // K quarter = word index of the prefix
   const float bound = (float)(128 - 2 * (int)threshold);
@@ -263,6 +322,162 @@ __device__ __forceinline__ void mfma_item(const uint32_t* __restrict__ lut, cons
                                     threshold, hits, capacity, counter, index_base);
 }
 
+// Shared-rows work item, for batches of kMfmaWaves or more query groups: the
+// block's waves stream the same kSharedTiles 16-row tiles (rows
+// sub * kSharedRows ...), wave w against query group kMfmaWaves * sgroup + w.
+// For every group of 4 tiles, wave w loads and expands tile 4g + w once, into
+// LDS (double-buffered, one barrier per group), and every wave reads the 4
+// expanded tiles with one 16-byte LDS read each.  Each wave would otherwise
+// load and expand every tile itself; shared, a product costs a quarter of the
+// loads and LUT reads, which is what more query tiles per wave would buy
+// without their registers.  A wave whose group lies past the last query still
+// loads and expands its tiles for the others.  The rare path is per wave, as
+// in mfma_item, reading its tiles' rows directly.
+constexpr int kSharedTiles = 64;
+constexpr uint64_t kSharedRows = (uint64_t)kSharedTiles * 16;
+static_assert(kSharedTiles % 4 == 0 && kSharedTiles / 4 <= 64, "one mask bit per group of 4 tiles");
+static_assert((uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 % kSharedRows == 0, "sub-chunks of a chunk");
+
+struct SharedRowsLds {
+  u32x4 ex[2][kMfmaWaves][64];  // two groups of kMfmaWaves expanded tiles, lane-major
+};
+
+// The shared-rows stream of one sub-chunk.  kFull (all kSharedTiles tiles
+// exist, every sub-chunk but a database's last) has a loop with fixed bounds
+// and no branch: a branch in the loop makes the compiler wait for every load
+// in flight at each group.
+template <int QT, bool kFull, bool kPlane>
+__device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const uint32_t* __restrict__ lut,
+                                              SharedRowsLds& S, const RowSource<kPlane>& src,
+                                              const uint32_t* __restrict__ db, uint64_t row0, uint64_t n_rows,
+                                              uint32_t qbase, int wave, int lane, float bound,
+                                              uint4* __restrict__ recs, uint32_t* __restrict__ rec_count,
+                                              uint32_t rec_cap) {
+  const int rn = lane & 15, kq = lane >> 4;
+  const uint64_t left = n_rows - row0;
+  const uint32_t tiles = kFull ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
+  const uint32_t ngroups = kFull ? (uint32_t)(kSharedTiles / 4) : (tiles + 3) / 4;
+  // This wave's word of tile 4g + wave, loaded unconditionally: the tile is
+  // clamped into the sub-chunk (the plane is padded to whole tiles) and a
+  // raw row past the end is clamped to the last row; stage() zeroes both.
+  auto word = [&](uint32_t g) -> uint32_t {
+    const uint32_t tt = min(4 * g + (uint32_t)wave, tiles - 1);
+    if (kPlane || kFull) return src(tt);
+    const uint64_t r = min((uint64_t)tt * 16 + rn, left - 1);
+    return db[(row0 + r) * 8 + kq];
+  };
+  auto stage = [&](uint32_t g, uint32_t w, int buf) {
+    const uint32_t tt = 4 * g + (uint32_t)wave;
+    Expanded e;
+    expand_word(w, lut, e);
+    // zero operands (not the expansion of a zero word, which is all -1),
+    // selected without a branch
+    const bool have = kFull ? tt < tiles : (tt < tiles && (uint64_t)tt * 16 + rn < left);
+    const uint32_t keep = have ? 0xffffffffu : 0u;
+    u32x4 v;
+    v.x = e.d[0] & keep;
+    v.y = e.d[1] & keep;
+    v.z = e.d[2] & keep;
+    v.w = e.d[3] & keep;
+    S.ex[buf][wave][lane] = v;
+  };
+  // ring[a] holds this wave's word of group g0 + a + 1, loaded kRing groups
+  // ahead; with the inner loop unrolled the slots are fixed registers
+  constexpr int kRing = 4;
+  static_assert((kSharedTiles / 4) % kRing == 0, "whole rounds of the ring");
+  uint32_t ring[kRing];
+#pragma unroll
+  for (int a = 0; a < kRing; a++) ring[a] = word(a + 1 < (int)ngroups ? a + 1 : ngroups - 1);
+  stage(0, word(0), 0);
+  __syncthreads();
+  int mx[kMaxChains];
+#pragma unroll
+  for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
+  const int ibound = __float_as_int(bound);
+  uint64_t pending = 0;  // groups of 4 tiles whose products reached the bound
+#pragma unroll 1
+  for (uint32_t g0 = 0; g0 < ngroups; g0 += kRing) {
+#pragma unroll
+    for (int a = 0; a < kRing; a++) {
+      const uint32_t g = g0 + a;
+      if (!kFull && g >= ngroups) break;  // the same for the whole block
+      // The buffer written here was last read before the previous barrier.
+      // After the last group this stages zeros that nobody reads.
+      if (kFull || g + 1 < ngroups) stage(g + 1, ring[a], (g + 1) & 1);
+      ring[a] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1);
+      // the group's 4 expanded tiles, read together so their latencies overlap
+      u32x4 vt[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) vt[j] = S.ex[g & 1][j][lane];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        Expanded rb;
+        rb.d[0] = vt[j].x;
+        rb.d[1] = vt[j].y;
+        rb.d[2] = vt[j].z;
+        rb.d[3] = vt[j].w;
+        tile_max<QT>(qa, rb, mx);
+      }
+      const int mall = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
+      // the vote is a wave-uniform mask, so the pending bits stay in scalar
+      // registers
+      const uint64_t vote = __ballot(mall >= ibound);
+      pending |= (uint64_t)(vote != 0) << g;
+#pragma unroll
+      for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
+      __syncthreads();
+    }
+  }
+  // The rare path is a record for the resolve kernel (sub-chunk, this
+  // wave's first query, the pending groups): checking pairs here, even in
+  // code that never runs, slows the loop by a tenth.
+  if (pending && lane == 0) {
+    const uint32_t slot = atomicAdd(rec_count, 1u);
+    const uint64_t sub = row0 / kSharedRows;
+    if (slot < rec_cap) recs[slot] = make_uint4((uint32_t)sub, (uint32_t)(sub >> 32), qbase, (uint32_t)pending);
+  }
+}
+
+template <int QT, bool kPlane>
+__device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lut, SharedRowsLds& S,
+                                                 const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane,
+                                                 uint64_t n_rows, uint64_t sub, uint32_t sgroup,
+                                                 const uint32_t* __restrict__ queries, uint32_t n_queries,
+                                                 uint32_t threshold, uint4* __restrict__ recs,
+                                                 uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
+  static_assert(kMfmaWaves == 4, "a group of 4 tiles, one per wave");
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: the tests on it are scalar
+  const int rn = lane & 15, kq = lane >> 4;
+  const float bound = (float)(128 - 2 * (int)threshold);
+  const uint64_t row0 = sub * kSharedRows;
+  if (row0 >= n_rows) return;  // the same for every wave of the block
+
+  // query operands; a wave whose group lies past the last query gets zero
+  // operands (its products never reach the bound) and still stages its tiles
+  const uint32_t qbase = (sgroup * kMfmaWaves + wave) * (uint32_t)(QT * 16);
+  uint32_t qw[QT];
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    const uint32_t qi = qbase + 16 * t + rn;
+    qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+  }
+  Expanded qa[QT];
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    expand_word(qw[t], lut, qa[t]);
+    if (qbase + 16 * t + rn >= n_queries) qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;
+  }
+
+  const RowSource<kPlane> src(db, plane, row0, lane);
+  if (row0 + kSharedRows <= n_rows)
+    stream_shared<QT, true, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
+                                    rec_cap);
+  else
+    stream_shared<QT, false, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
+                                     rec_cap);
+}
+
 __device__ __forceinline__ void init_lut(uint32_t* lut) {
   lut[threadIdx.x] = spread_byte(threadIdx.x);  // blockDim.x == 256
   __syncthreads();
@@ -279,19 +494,152 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
   group = (uint32_t)(slot % groups);
 }
 
-// One launch, one block per work item (row chunks x query groups).
-template <int QT, bool kPlane>
+// One launch, one block per work item: row chunks x query groups (the
+// per-wave layout, which checks its candidates itself), or, with kShared
+// (4+ query groups), 1024-row sub-chunks x groups of kMfmaWaves query groups,
+// whose candidates become records for pdq_match_resolve_kernel.
+template <int QT, bool kPlane, bool kShared>
 __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
-    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups) {
+    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups,
+    uint4* __restrict__ recs, uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
   __shared__ uint32_t lut[256];
   init_lut(lut);
   uint64_t chunk;
   uint32_t group;
   item_coords(blockIdx.x, groups, chunk, group);
-  mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity, counter,
-                        index_base);
+  if constexpr (kShared) {
+    __shared__ SharedRowsLds S;
+    mfma_item_shared<QT, kPlane>(lut, S, db, plane, n_rows, chunk, group, queries, n_queries, threshold, recs,
+                                 rec_count, rec_cap);
+  } else {
+    mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
+                          counter, index_base, (int)(threadIdx.x >> 6));
+  }
+}
+
+// The second kernel of the shared-rows layout, in blocks of kResolveWaves
+// waves.  Each wave takes records in turn: for every pending group of the
+// record's sub-chunk it recomputes the 4 tiles' products, and for each query
+// tile with a pair at the bound checks every such pair with the exact
+// 256-bit distance.  The block's hits gather in LDS and take their slots in
+// the hit buffer with one atomic.  If the records overflowed their area (more
+// than rec_cap: dense hits), every work item is matched again in the
+// per-wave layout instead (each half of the block an item), which checks its
+// own candidates (the first kernel appended no hits), so the set stays exact.
+constexpr int kResolveWaves = 8;
+static_assert(kResolveWaves % kMfmaWaves == 0, "whole per-wave items per block");
+
+template <int QT, bool kPlane>
+__global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
+    const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
+    const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
+    uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base,
+    const uint4* __restrict__ recs, const uint32_t* __restrict__ rec_count, uint32_t rec_cap, uint64_t chunks,
+    uint32_t groups) {
+  __shared__ uint32_t lut[256];
+  __shared__ MatchHit stage[kStageHits];
+  __shared__ uint32_t n_stage;
+  __shared__ unsigned long long stage_base;
+  if (threadIdx.x < 256) lut[threadIdx.x] = spread_byte(threadIdx.x);
+  if (threadIdx.x == 0) n_stage = 0;
+  __syncthreads();
+  const int wave = (int)(threadIdx.x >> 6);
+  const uint32_t n_recs = *rec_count;
+  if (n_recs > rec_cap) {
+    constexpr int kHalves = kResolveWaves / kMfmaWaves;
+    const uint64_t items = chunks * groups;
+    for (uint64_t item = (uint64_t)blockIdx.x * kHalves + wave / kMfmaWaves; item < items;
+         item += (uint64_t)gridDim.x * kHalves) {
+      uint64_t chunk;
+      uint32_t group;
+      item_coords(item, groups, chunk, group);
+      mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
+                            counter, index_base, wave % kMfmaWaves);
+    }
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int rn = lane & 15, kq = lane >> 4;
+  const float bound = (float)(128 - 2 * (int)threshold);
+  const int ibound = __float_as_int(bound);
+  const uint32_t stride = gridDim.x * kResolveWaves;
+  uint32_t r = blockIdx.x * kResolveWaves + wave;
+  // A record's loads are issued together (its query words, then each pending
+  // group's 4 tile words) and the next record is fetched while this one
+  // runs: the pass is a few dependent loads per record, not one per tile.
+  uint4 rec = r < n_recs ? recs[r] : make_uint4(0, 0, 0, 0);
+  for (; r < n_recs; r += stride) {
+    const uint4 next = r + stride < n_recs ? recs[r + stride] : make_uint4(0, 0, 0, 0);
+    const uint64_t row0 = ((uint64_t)rec.y << 32 | rec.x) * kSharedRows;
+    const uint32_t qbase = rec.z;
+    uint32_t pending = rec.w;
+    const uint64_t left = n_rows - row0;
+    const uint32_t tiles = left >= kSharedRows ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
+    uint32_t qw[QT];
+#pragma unroll
+    for (int t = 0; t < QT; t++) {
+      const uint32_t qi = qbase + 16 * t + rn;
+      qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+    }
+    Expanded qa[QT];
+#pragma unroll
+    for (int t = 0; t < QT; t++) {
+      expand_word(qw[t], lut, qa[t]);
+      if (qbase + 16 * t + rn >= n_queries) qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;
+    }
+    const RowSource<kPlane> src(db, plane, row0, lane);
+    while (pending) {
+      const uint32_t g = (uint32_t)__builtin_ctz(pending);
+      pending &= pending - 1;
+      uint32_t w[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        // clamped into the sub-chunk's tiles (and, for raw rows, its rows)
+        const uint32_t tt = min(4 * g + (uint32_t)j, tiles - 1);
+        if (kPlane) {
+          w[j] = src(tt);
+        } else {
+          const uint64_t rr = min((uint64_t)tt * 16 + rn, left - 1);
+          w[j] = db[(row0 + rr) * 8 + kq];
+        }
+      }
+#pragma unroll 1
+      for (int j = 0; j < 4; j++) {
+        const uint32_t tt = 4 * g + (uint32_t)j;
+        if (tt >= tiles) break;  // the same for the whole wave
+        Expanded eg;
+        expand_word(j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3], lut, eg);
+        if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
+        // which query tiles have a pair at the bound (a wave-uniform mask)
+        uint32_t tmask = 0;
+#pragma unroll
+        for (int t = 0; t < QT; t++) {
+          const v4f c = mfma_prefix(qa[t], eg);
+          const int m = max(max(__float_as_int(c.x), __float_as_int(c.y)), max(__float_as_int(c.z), __float_as_int(c.w)));
+          tmask |= (wave_any(m >= ibound) ? 1u : 0u) << t;
+        }
+        // only those tiles' products are recomputed and checked, out of line:
+        // QT inlined checks keep several hundred registers live
+#pragma unroll
+        for (int t = 0; t < QT; t++)
+          if (tmask >> t & 1u)
+            stage_candidates(mfma_prefix(qa[t], eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
+                             n_queries, n_rows, threshold, stage, &n_stage, hits, capacity, counter, index_base);
+      }
+    }
+    rec = next;
+  }
+  // the block's hits into the buffer: one run of slots
+  __syncthreads();
+  const uint32_t n = min(n_stage, kStageHits);
+  if (n == 0) return;  // the same for the whole block
+  if (threadIdx.x == 0) stage_base = atomicAdd(counter, (unsigned long long)n);
+  __syncthreads();
+  const unsigned long long base = stage_base;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    if (base + i < capacity) hits[base + i] = stage[i];
 }
 
 // Persistent form over a set of database shards: a fixed grid (a multiple of
@@ -320,30 +668,113 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
     const ShardView& v = shards[sh];
     mfma_item<QT, kPlane>(lut, reinterpret_cast<const uint32_t*>(v.rows), reinterpret_cast<const uint32_t*>(v.plane),
                           v.n_rows, chunk - v.chunk_begin, group, queries, nq, threshold, hits, capacity, counter,
-                          v.index_base);
+                          v.index_base, (int)(threadIdx.x >> 6));
   }
 }
+
+// Record areas of the shared-rows layout: one per (device, stream), made on
+// the stream's first such launch and kept for the life of the process, with
+// a mutex so that the count reset, the filter and the resolve kernel of one
+// call are enqueued together even when threads share a stream.  A stream
+// being captured into a graph cannot allocate: those calls run the per-wave
+// layout.
+constexpr uint32_t kRecordCap = 65536;  // 1 MiB: dense-hit batches overflow into a per-wave rerun
+
+struct RecordArea {
+  uint4* recs = nullptr;
+  uint32_t* count = nullptr;
+  int cus = 0;  // compute units of the area's device
+};
+
+static std::mutex g_record_mu;
+
+static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable) {
+  *usable = false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(stream, &cs);
+  if (e != hipSuccess) return e;
+  if (cs != hipStreamCaptureStatusNone) return hipSuccess;
+  int dev = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  static std::map<std::pair<int, hipStream_t>, RecordArea> areas;  // guarded by g_record_mu
+  auto it = areas.find({dev, stream});
+  if (it == areas.end()) {
+    void* p = nullptr;
+    if ((e = hipMalloc(&p, (size_t)kRecordCap * sizeof(uint4) + 256)) != hipSuccess) return e;
+    RecordArea ra;
+    if ((e = hipDeviceGetAttribute(&ra.cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) {
+      (void)hipFree(p);
+      return e;
+    }
+    ra.recs = static_cast<uint4*>(p);
+    ra.count = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(p) + (size_t)kRecordCap * sizeof(uint4));
+    it = areas.emplace(std::make_pair(dev, stream), ra).first;
+  }
+  *out = it->second;
+  *usable = true;
+  return hipSuccess;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
 template <int QT>
 static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                             uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                             unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
-  const uint64_t rows_per_block = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
-  const uint64_t chunks = (n_rows + rows_per_block - 1) / rows_per_block;
-  const uint64_t chunks8 = (chunks + 7) / 8 * 8;  // whole rounds over the 8 XCDs
   const uint32_t groups = (n_queries + QT * 16 - 1) / (QT * 16);
+  const uint32_t* d = static_cast<const uint32_t*>(db);
+  const uint32_t* p = static_cast<const uint32_t*>(plane);
+  const uint32_t* q = static_cast<const uint32_t*>(queries);
+  const dim3 block(64 * kMfmaWaves);
+  // the per-wave layout's items: 4096-row chunks x query groups
+  const uint64_t rows_per_chunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
+  const uint64_t chunks8 = ((n_rows + rows_per_chunk - 1) / rows_per_chunk + 7) / 8 * 8;  // whole XCD rounds
+  if (groups >= (uint32_t)kMfmaWaves) {
+    std::lock_guard<std::mutex> g(g_record_mu);
+    RecordArea ra;
+    bool usable = false;
+    hipError_t e = record_area(stream, &ra, &usable);
+    if (e != hipSuccess) return e;
+    if (usable) {
+      const uint64_t subs8 = ((n_rows + kSharedRows - 1) / kSharedRows + 7) / 8 * 8;
+      const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
+      const uint64_t blocks = subs8 * sgroups;
+      if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, ra.count);
+      if (plane)
+        hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
+                           ra.count, kRecordCap);
+      else
+        hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
+                           ra.count, kRecordCap);
+      // one block per compute unit: plenty for the few records of sparse
+      // hits, one hit-slot atomic per block, and the per-wave rerun after an
+      // overflow walks all items with them
+      const uint64_t rgrid = std::min<uint64_t>((uint64_t)ra.cus, chunks8 * groups);
+      const dim3 rblock(64 * kResolveWaves);
+      if (plane)
+        hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, true>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
+                           kRecordCap, chunks8, groups);
+      else
+        hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, false>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
+                           kRecordCap, chunks8, groups);
+      return hipGetLastError();
+    }
+  }
   const uint64_t blocks = chunks8 * groups;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   if (plane)
-    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
-                       static_cast<const uint32_t*>(db), static_cast<const uint32_t*>(plane), n_rows,
-                       static_cast<const uint32_t*>(queries), n_queries, threshold, hits, capacity, counter,
-                       index_base, groups);
+    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, false>), dim3((unsigned)blocks), block, 0, stream, d, p,
+                       n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, groups, nullptr, nullptr,
+                       0u);
   else
-    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false>), dim3((unsigned)blocks), dim3(64 * kMfmaWaves), 0, stream,
-                       static_cast<const uint32_t*>(db), static_cast<const uint32_t*>(plane), n_rows,
-                       static_cast<const uint32_t*>(queries), n_queries, threshold, hits, capacity, counter,
-                       index_base, groups);
+    hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, false>), dim3((unsigned)blocks), block, 0, stream, d, p,
+                       n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, groups, nullptr, nullptr,
+                       0u);
   return hipGetLastError();
 }
 
