@@ -286,8 +286,9 @@ def main():
         ops = 256.0 * n_rows * nq_head
         achieved = ops / (t_match_head * 1e-3) / 1e12
         roofline_match = {
-            "kernel": (f"pdq_match_mfma_kernel (fp4 MFMA 16x16x128 prefix filter over the prefix "
-                       f"plane + exact 256-bit recheck): {n_rows} rows x {nq_head} queries, thr {thr}"),
+            "kernel": (f"matrix-core matcher call over the prefix plane: pdq_match_mfma_kernel (fp4 MFMA "
+                       f"16x16x128 prefix filter, waves sharing rows) + pdq_match_resolve_kernel (exact "
+                       f"256-bit recheck of candidate groups): {n_rows} rows x {nq_head} queries, thr {thr}"),
             "bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_FP4_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / MFMA_FP4_PEAK_TFLOPS, 4),
             "traffic": counters.get(f"match_plane_x{nq_head}", {}).get("traffic_bytes"),
