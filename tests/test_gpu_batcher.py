@@ -184,3 +184,68 @@ def test_shards_load_raw(gpu, dev, tmp_path):
         assert n1 == n2 and np.array_equal(triples(h1), triples(h2))
     finally:
         s.close()
+
+
+def test_batcher_shared_rows_planted(gpu, dev):
+    """Batches of 4 or more query groups (64, 100 and 128 images: 512, 800
+    and 1024 queries) replay the shared-rows graphs: a block's waves stream
+    the same rows, candidates go through records to the recheck kernel.
+    Ragged shards (5003, 1, 70,000 and 300,000 rows) hold near-duplicates of
+    the batch's own hashes; each batch equals the single-database matcher
+    over the concatenation, with and without the graph."""
+    import torch
+    imgs = torch.from_numpy(synthetic.images_u8(128)).to(dev)
+    hq = gpu.hash_batch(imgs)[0].reshape(-1, 32).cpu().numpy()
+    parts = []
+    for k, n in enumerate((5003, 1, 70_000, 300_000)):
+        d = synthetic.random_db(n, seed=300 + k)
+        synthetic.plant_fast(d, hq, max(1, n // 50), seed=400 + k)
+        parts.append(d)
+    s = gpu.Shards()
+    for d in parts:
+        s.add(d)
+    full = torch.from_numpy(np.concatenate(parts)).to(dev)
+    b = gpu.Batcher(s, 128, threshold=31, hit_capacity=1 << 20)
+    try:
+        for n in (128, 64, 100, 128):
+            say(f"[batcher] shared rows, batch of {n}")
+            planes = imgs[:n]
+            h, _, _, total, hits = b.run(planes)
+            en, ehits = gpu.match(full, h.reshape(-1, 32), 31, capacity=1 << 20)
+            assert total == en and en > 0
+            assert np.array_equal(triples(hits), triples(ehits))
+            _, _, _, total2, hits2 = b.run(planes, use_graph=False)
+            assert total2 == en and np.array_equal(triples(hits2), triples(ehits))
+    finally:
+        b.close()
+        s.close()
+
+
+def test_batcher_shared_rows_record_overflow(gpu, dev, monkeypatch):
+    """256 images (2048 queries) at threshold 63 over 1M rows in two shards:
+    every group of tiles is a candidate, 977 sub-chunks x 16 query groups,
+    more records than the 4096 allowed here (YUME_PDQ_AMD_MATCH_RECORD_CAP,
+    read when the graph is captured), so the captured recheck kernel matches
+    every item again per wave.  The set equals the single-database
+    vector-ALU matcher's."""
+    import torch
+    monkeypatch.setenv("YUME_PDQ_AMD_MATCH_RECORD_CAP", "4096")
+    imgs = torch.from_numpy(synthetic.images_u8(256)).to(dev)
+    hq = gpu.hash_batch(imgs)[0].reshape(-1, 32).cpu().numpy()
+    parts = [synthetic.random_db(n, seed=500 + k) for k, n in enumerate((600_000, 400_001))]
+    for k, d in enumerate(parts):
+        synthetic.plant_fast(d, hq, 5000, seed=600 + k)
+    s = gpu.Shards()
+    for d in parts:
+        s.add(d)
+    full = torch.from_numpy(np.concatenate(parts)).to(dev)
+    b = gpu.Batcher(s, 256, threshold=63, hit_capacity=1 << 20)
+    try:
+        say("[batcher] record overflow, graph")
+        h, _, _, total, hits = b.run(imgs)
+        en, ehits = gpu.match(full, h.reshape(-1, 32), 63, capacity=1 << 20, path="valu")
+        assert total == en and en >= 9000
+        assert np.array_equal(triples(hits), triples(ehits))
+    finally:
+        b.close()
+        s.close()

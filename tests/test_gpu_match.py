@@ -164,23 +164,26 @@ def test_mfma_every_pair_a_hit(gpu, oracle, dev, plane):
     assert set(map(tuple, t.tolist())) <= set(map(tuple, exp.tolist()))
 
 
-def test_mfma_candidate_records_overflow(gpu, dev):
-    """5M rows x 2048 queries at threshold 63: 4883 sub-chunks x 16 query
-    groups, every one a candidate, are more candidate records than the 65,536
-    the first pass keeps, so the second pass matches every work item again
+def test_mfma_candidate_records_overflow(gpu, dev, monkeypatch):
+    """1,000,003 rows x 2048 queries at threshold 63: 977 sub-chunks x 16
+    query groups, every one a candidate, are more candidate records than the
+    4096 the first pass may keep here (YUME_PDQ_AMD_MATCH_RECORD_CAP; the
+    default is 1,048,576), so the second pass matches every work item again
     with per-wave candidate checks.  The set equals the vector-ALU kernel's,
     and holds the planted pairs."""
     import torch
-    db = synthetic.random_db(5_000_000, seed=93)
+    monkeypatch.setenv("YUME_PDQ_AMD_MATCH_RECORD_CAP", "4096")
+    db = synthetic.random_db(1_000_003, seed=93)
     q = synthetic.random_db(2048, seed=94)
     synthetic.plant_fast(db, q, 20_000, seed=95)
     d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
-    say("[match] record overflow: mfma")
-    n1, h1 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="mfma")
-    say("[match] record overflow: valu")
-    n2, h2 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="valu")
-    assert n1 == n2 and np.array_equal(as_triples(h1), as_triples(h2))
-    assert n1 >= 0.9 * 20_000
+    for plane in (None, gpu.build_prefix_plane(d_db)):
+        say(f"[match] record overflow: mfma plane={plane is not None}")
+        n1, h1 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="mfma", plane=plane)
+        say("[match] record overflow: valu")
+        n2, h2 = gpu.match(d_db, d_q, 63, capacity=1 << 20, path="valu")
+        assert n1 == n2 and np.array_equal(as_triples(h1), as_triples(h2))
+        assert n1 >= 0.9 * 20_000
 
 
 def test_prefix_plane_layout(gpu, dev):

@@ -15,10 +15,12 @@
 namespace ypdq {
 
 // Which matcher a batch uses: the vector-ALU scan, or the matrix-core filter
-// with qt 16-query tiles per wave.
+// with qt 16-query tiles per wave, in the per-wave layout or (shared, with a
+// record area) with a block's waves sharing their rows.
 struct MatcherChoice {
   bool mfma = false;
   int qt = 8;
+  bool shared = false;
 };
 
 struct TuneRecord {
@@ -50,10 +52,12 @@ class ShardSet {
                      unsigned long long* d_bad, hipStream_t stream);
 
   // One persistent launch over every shard.  d_nq: device query count (or
-  // null for nq_cap).
+  // null for nq_cap).  A shared-rows choice needs record_area
+  // (match_record_area_bytes(), used by nothing else in flight); without it
+  // the per-wave layout runs.
   hipError_t launch_match(const MatcherChoice& c, const void* d_queries, const uint32_t* d_nq, uint32_t nq_cap,
                           uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                          hipStream_t stream) const;
+                          hipStream_t stream, void* record_area = nullptr) const;
 
   uint32_t count() const { return (uint32_t)shards_.size(); }
   uint64_t total_rows() const;
@@ -146,6 +150,7 @@ class BatchContext {
   unsigned long long* d_count_ = nullptr;
   uint32_t* d_sizes_ = nullptr;  // device {n_images, n_queries}
   uint32_t* h_sizes_ = nullptr;  // pinned staging of the same
+  void* d_records_ = nullptr;    // the shared-rows matcher's candidate records
   std::map<uint32_t, MatcherChoice> choices_;  // first query count of a bucket -> choice
   std::map<int, Graph> graphs_;
 };

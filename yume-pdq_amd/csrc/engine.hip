@@ -181,9 +181,21 @@ uint64_t ShardSet::device_bytes() const {
 
 hipError_t ShardSet::launch_match(const MatcherChoice& c, const void* d_queries, const uint32_t* d_nq, uint32_t nq_cap,
                                   uint32_t threshold, MatchHit* hits, uint64_t capacity, unsigned long long* counter,
-                                  hipStream_t stream) const {
+                                  hipStream_t stream, void* record_area) const {
   if (shards_.empty()) return hipSuccess;
   const int cus = cus_;
+  if (c.mfma && c.shared && record_area) {
+    // resident blocks of the streaming kernel as below; the recheck kernel
+    // runs one 8-wave block per CU
+    uint64_t grid = (uint64_t)cus * (c.qt >= 16 ? 2 : 4);
+    const uint64_t groups = (nq_cap + c.qt * 16 - 1) / (c.qt * 16);
+    const uint64_t items = total_chunks_ * 4 * ((groups + 3) / 4);
+    if (grid > items) grid = (items + 7) / 8 * 8;
+    grid = std::max<uint64_t>(8, grid / 8 * 8);
+    return launch_match_mfma_shards_shared(c.qt, d_views_, (uint32_t)shards_.size(), total_chunks_, true, d_queries,
+                                           d_nq, nq_cap, threshold, hits, capacity, counter, record_area,
+                                           (uint32_t)grid, (uint32_t)std::max(1, cus), stream);
+  }
   if (c.mfma) {
     // resident blocks: 4 waves per SIMD at 2..8 query tiles, 2 at 16
     uint64_t grid = (uint64_t)cus * (c.qt >= 16 ? 2 : 4);
@@ -210,7 +222,7 @@ BatchContext::~BatchContext() {
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
   }
-  void* bufs[] = {d_input_, d_out_, d_sizes_};
+  void* bufs[] = {d_input_, d_out_, d_sizes_, d_records_};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h_sizes_) (void)hipHostFree(h_sizes_);
@@ -239,6 +251,7 @@ hipError_t BatchContext::init(std::shared_ptr<ShardSet> shards, uint32_t max_ima
   d_threshold_ = reinterpret_cast<float*>(d_out_ + out_offset(kOutThreshold));
   d_hits_ = d_out_ + out_offset(kOutHits);
   if ((e = hipMalloc(&d_sizes_, 2 * sizeof(uint32_t))) != hipSuccess) return e;
+  if ((e = hipMalloc(&d_records_, match_record_area_bytes())) != hipSuccess) return e;
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_sizes_), 2 * sizeof(uint32_t), hipHostMallocDefault)) !=
       hipSuccess)
     return e;
@@ -259,21 +272,30 @@ uint64_t BatchContext::out_offset(OutPart p) const {
   }
 }
 
-uint64_t BatchContext::device_bytes() const { return plane_stride_ * max_images_ + out_bytes(hit_capacity_) + 16; }
+uint64_t BatchContext::device_bytes() const {
+  return plane_stride_ * max_images_ + out_bytes(hit_capacity_) + 16 + match_record_area_bytes();
+}
+
+// Rows shared by a block's waves pay off from 4 query groups (one per wave).
+static bool shared_pays(const MatcherChoice& c, uint32_t nq) {
+  return c.mfma && (nq + c.qt * 16 - 1) / (c.qt * 16) >= 4;
+}
 
 MatcherChoice BatchContext::choice_for(uint32_t nq) const {
   auto it = choices_.upper_bound(nq);
+  MatcherChoice c;
   if (it != choices_.begin()) {
     --it;
-    return it->second;
+    c = it->second;
+  } else {
+    c.mfma = threshold_ < 64 && nq >= 32;
+    c.qt = 8;
   }
-  MatcherChoice c;
-  c.mfma = threshold_ < 64 && nq >= 32;
-  c.qt = 8;
+  c.shared = shared_pays(c, nq);
   return c;
 }
 
-static int choice_key(const MatcherChoice& c) { return c.mfma ? c.qt : 0; }
+static int choice_key(const MatcherChoice& c) { return c.mfma ? c.qt * 2 + (c.shared ? 1 : 0) : 0; }
 
 hipError_t BatchContext::enqueue_body(const MatcherChoice& c, hipStream_t s) {
   HashOutputs o;
@@ -286,7 +308,7 @@ hipError_t BatchContext::enqueue_body(const MatcherChoice& c, hipStream_t s) {
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, d_count_, 0ull);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return shards_->launch_match(c, d_hashes_, d_sizes_ + 1, max_images_ * 8, threshold_,
-                               reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, s);
+                               reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, s, d_records_);
 }
 
 hipError_t BatchContext::graph_for(const MatcherChoice& c, hipGraphExec_t* exec) {
@@ -375,6 +397,7 @@ hipError_t BatchContext::autotune(int iterations, std::vector<TuneRecord>* recor
         MatcherChoice m;
         m.mfma = true;
         m.qt = qt;
+        m.shared = shared_pays(m, nq);
         cands.push_back(m);
       }
     float best = 1e30f;
@@ -388,7 +411,8 @@ hipError_t BatchContext::autotune(int iterations, std::vector<TuneRecord>* recor
       auto launch = [&]() -> hipError_t {
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, stream_, d_count_, 0ull);
         return shards_->launch_match(c, d_hashes_, d_sizes_ + 1, nq, threshold_,
-                                     reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, stream_);
+                                     reinterpret_cast<MatchHit*>(d_hits_), hit_capacity_, d_count_, stream_,
+                                     d_records_);
       };
       if ((e = launch()) != hipSuccess) return e;  // warm-up
       if ((e = hipEventRecord(a, stream_)) != hipSuccess) return e;

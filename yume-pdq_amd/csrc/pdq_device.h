@@ -114,6 +114,16 @@ hipError_t launch_match_mfma_shards(int qt, const ShardView* shards, uint32_t n_
                                     bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
                                     uint32_t min_nq, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                                     unsigned long long* counter, uint32_t grid, hipStream_t stream);
+// The shared-rows layout over a set of shards (4 or more query groups pay
+// off): record-count reset, persistent streaming kernel, recheck kernel.
+// record_area: match_record_area_bytes() of device memory used by no other
+// launch in flight (a batch context's own).
+uint64_t match_record_area_bytes();
+hipError_t launch_match_mfma_shards_shared(int qt, const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
+                                           bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                           uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                           unsigned long long* counter, void* record_area, uint32_t grid,
+                                           uint32_t rgrid, hipStream_t stream);
 hipError_t launch_match_valu_shards(const MatchTuning& t, const ShardView* shards, uint32_t n_shards,
                                     uint64_t total_rblocks, bool plane, const void* queries, const uint32_t* d_nq,
                                     uint32_t nq_cap, uint32_t max_nq, uint32_t threshold, MatchHit* hits,

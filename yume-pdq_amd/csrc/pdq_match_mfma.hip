@@ -20,6 +20,7 @@
 // (m & 3).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <map>
@@ -118,9 +119,10 @@ __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, u
 // The shared-rows layout's second kernel appends to its block's hit list in
 // LDS (stage, n_stage) and takes its slots in the hit buffer once per block:
 // thousands of appends to the one global counter from a short kernel
-// serialise on it (measured: 49 of 68 microseconds for 4938 hits).  Hits past
-// the list's kStageHits go straight to the buffer.
-constexpr uint32_t kStageHits = 512;
+// serialise on it (measured: 49 of 68 microseconds for 4938 hits).  The list
+// (64 KiB) holds 4096 hits: with one block per CU, about a million per
+// launch.  Hits past it go straight to the buffer.
+constexpr uint32_t kStageHits = 4096;
 
 __device__ __noinline__ void stage_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
                                               float bound, const uint32_t* __restrict__ queries,
@@ -337,6 +339,7 @@ constexpr int kSharedTiles = 64;
 constexpr uint64_t kSharedRows = (uint64_t)kSharedTiles * 16;
 static_assert(kSharedTiles % 4 == 0 && kSharedTiles / 4 <= 64, "one mask bit per group of 4 tiles");
 static_assert((uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 % kSharedRows == 0, "sub-chunks of a chunk");
+constexpr uint64_t kSubsPerChunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 / kSharedRows;
 
 struct SharedRowsLds {
   u32x4 ex[2][kMfmaWaves][64];  // two groups of kMfmaWaves expanded tiles, lane-major
@@ -352,7 +355,7 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
                                               const uint32_t* __restrict__ db, uint64_t row0, uint64_t n_rows,
                                               uint32_t qbase, int wave, int lane, float bound,
                                               uint4* __restrict__ recs, uint32_t* __restrict__ rec_count,
-                                              uint32_t rec_cap) {
+                                              uint32_t rec_cap, uint64_t rec_sub_base) {
   const int rn = lane & 15, kq = lane >> 4;
   const uint64_t left = n_rows - row0;
   const uint32_t tiles = kFull ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
@@ -428,12 +431,13 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
       __syncthreads();
     }
   }
-  // The rare path is a record for the resolve kernel (sub-chunk, this
-  // wave's first query, the pending groups): checking pairs here, even in
-  // code that never runs, slows the loop by a tenth.
+  // The rare path is a record for the resolve kernel (sub-chunk number, plus
+  // rec_sub_base for a shard of a set; this wave's first query; the pending
+  // groups): checking pairs here, even in code that never runs, slows the
+  // loop by a tenth.
   if (pending && lane == 0) {
     const uint32_t slot = atomicAdd(rec_count, 1u);
-    const uint64_t sub = row0 / kSharedRows;
+    const uint64_t sub = rec_sub_base + row0 / kSharedRows;
     if (slot < rec_cap) recs[slot] = make_uint4((uint32_t)sub, (uint32_t)(sub >> 32), qbase, (uint32_t)pending);
   }
 }
@@ -444,7 +448,8 @@ __device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lu
                                                  uint64_t n_rows, uint64_t sub, uint32_t sgroup,
                                                  const uint32_t* __restrict__ queries, uint32_t n_queries,
                                                  uint32_t threshold, uint4* __restrict__ recs,
-                                                 uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
+                                                 uint32_t* __restrict__ rec_count, uint32_t rec_cap,
+                                                 uint64_t rec_sub_base) {
   static_assert(kMfmaWaves == 4, "a group of 4 tiles, one per wave");
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: the tests on it are scalar
@@ -472,10 +477,10 @@ __device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lu
   const RowSource<kPlane> src(db, plane, row0, lane);
   if (row0 + kSharedRows <= n_rows)
     stream_shared<QT, true, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
-                                    rec_cap);
+                                    rec_cap, rec_sub_base);
   else
     stream_shared<QT, false, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
-                                     rec_cap);
+                                     rec_cap, rec_sub_base);
 }
 
 __device__ __forceinline__ void init_lut(uint32_t* lut) {
@@ -512,7 +517,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
   if constexpr (kShared) {
     __shared__ SharedRowsLds S;
     mfma_item_shared<QT, kPlane>(lut, S, db, plane, n_rows, chunk, group, queries, n_queries, threshold, recs,
-                                 rec_count, rec_cap);
+                                 rec_count, rec_cap, 0);
   } else {
     mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
                           counter, index_base, (int)(threadIdx.x >> 6));
@@ -531,6 +536,104 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 constexpr int kResolveWaves = 8;
 static_assert(kResolveWaves % kMfmaWaves == 0, "whole per-wave items per block");
 
+struct ResolveLds {
+  uint32_t lut[256];
+  MatchHit stage[kStageHits];
+  uint32_t n_stage;
+  unsigned long long stage_base;
+};
+
+__device__ __forceinline__ void resolve_init(ResolveLds& L) {
+  if (threadIdx.x < 256) L.lut[threadIdx.x] = spread_byte(threadIdx.x);
+  if (threadIdx.x == 0) L.n_stage = 0;
+  __syncthreads();
+}
+
+// The block's gathered hits into the buffer: one run of slots.  Every thread
+// of the block calls it.
+__device__ __forceinline__ void resolve_flush(ResolveLds& L, MatchHit* __restrict__ hits, uint64_t capacity,
+                                              unsigned long long* __restrict__ counter) {
+  __syncthreads();
+  const uint32_t n = min(L.n_stage, kStageHits);
+  if (n == 0) return;  // the same for the whole block
+  if (threadIdx.x == 0) L.stage_base = atomicAdd(counter, (unsigned long long)n);
+  __syncthreads();
+  const unsigned long long base = L.stage_base;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    if (base + i < capacity) hits[base + i] = L.stage[i];
+}
+
+// One record, checked by one wave: sub-chunk rows row0 ... of a database of
+// n_rows rows (global index index_base + row), queries qbase ...
+// (QT x 16), pending groups of 4 tiles.  Its loads are issued together (the
+// query words, then each pending group's 4 tile words).
+template <int QT, bool kPlane>
+__device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __restrict__ db,
+                                               const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t row0,
+                                               uint32_t qbase, uint32_t pending, const uint32_t* __restrict__ queries,
+                                               uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
+                                               uint64_t capacity, unsigned long long* __restrict__ counter,
+                                               uint64_t index_base) {
+  const int lane = threadIdx.x & 63;
+  const int rn = lane & 15, kq = lane >> 4;
+  const float bound = (float)(128 - 2 * (int)threshold);
+  const int ibound = __float_as_int(bound);
+  const uint64_t left = n_rows - row0;
+  const uint32_t tiles = left >= kSharedRows ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
+  uint32_t qw[QT];
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    const uint32_t qi = qbase + 16 * t + rn;
+    qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+  }
+  Expanded qa[QT];
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    expand_word(qw[t], L.lut, qa[t]);
+    if (qbase + 16 * t + rn >= n_queries) qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;
+  }
+  const RowSource<kPlane> src(db, plane, row0, lane);
+  while (pending) {
+    const uint32_t g = (uint32_t)__builtin_ctz(pending);
+    pending &= pending - 1;
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      // clamped into the sub-chunk's tiles (and, for raw rows, its rows)
+      const uint32_t tt = min(4 * g + (uint32_t)j, tiles - 1);
+      if (kPlane) {
+        w[j] = src(tt);
+      } else {
+        const uint64_t rr = min((uint64_t)tt * 16 + rn, left - 1);
+        w[j] = db[(row0 + rr) * 8 + kq];
+      }
+    }
+#pragma unroll 1
+    for (int j = 0; j < 4; j++) {
+      const uint32_t tt = 4 * g + (uint32_t)j;
+      if (tt >= tiles) break;  // the same for the whole wave
+      Expanded eg;
+      expand_word(j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3], L.lut, eg);
+      if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
+      // which query tiles have a pair at the bound (a wave-uniform mask)
+      uint32_t tmask = 0;
+#pragma unroll
+      for (int t = 0; t < QT; t++) {
+        const v4f c = mfma_prefix(qa[t], eg);
+        const int m = max(max(__float_as_int(c.x), __float_as_int(c.y)), max(__float_as_int(c.z), __float_as_int(c.w)));
+        tmask |= (wave_any(m >= ibound) ? 1u : 0u) << t;
+      }
+      // only those tiles' products are recomputed and checked, out of line:
+      // QT inlined checks keep several hundred registers live
+#pragma unroll
+      for (int t = 0; t < QT; t++)
+        if (tmask >> t & 1u)
+          stage_candidates(mfma_prefix(qa[t], eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
+                           n_queries, n_rows, threshold, L.stage, &L.n_stage, hits, capacity, counter, index_base);
+    }
+  }
+}
+
 template <int QT, bool kPlane>
 __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
@@ -538,13 +641,8 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
     uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base,
     const uint4* __restrict__ recs, const uint32_t* __restrict__ rec_count, uint32_t rec_cap, uint64_t chunks,
     uint32_t groups) {
-  __shared__ uint32_t lut[256];
-  __shared__ MatchHit stage[kStageHits];
-  __shared__ uint32_t n_stage;
-  __shared__ unsigned long long stage_base;
-  if (threadIdx.x < 256) lut[threadIdx.x] = spread_byte(threadIdx.x);
-  if (threadIdx.x == 0) n_stage = 0;
-  __syncthreads();
+  __shared__ ResolveLds L;
+  resolve_init(L);
   const int wave = (int)(threadIdx.x >> 6);
   const uint32_t n_recs = *rec_count;
   if (n_recs > rec_cap) {
@@ -555,91 +653,109 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
       uint64_t chunk;
       uint32_t group;
       item_coords(item, groups, chunk, group);
-      mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
+      mfma_item<QT, kPlane>(L.lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
                             counter, index_base, wave % kMfmaWaves);
     }
     return;
   }
-  const int lane = threadIdx.x & 63;
-  const int rn = lane & 15, kq = lane >> 4;
-  const float bound = (float)(128 - 2 * (int)threshold);
-  const int ibound = __float_as_int(bound);
   const uint32_t stride = gridDim.x * kResolveWaves;
   uint32_t r = blockIdx.x * kResolveWaves + wave;
-  // A record's loads are issued together (its query words, then each pending
-  // group's 4 tile words) and the next record is fetched while this one
-  // runs: the pass is a few dependent loads per record, not one per tile.
+  // the next record is fetched while this one runs
   uint4 rec = r < n_recs ? recs[r] : make_uint4(0, 0, 0, 0);
   for (; r < n_recs; r += stride) {
     const uint4 next = r + stride < n_recs ? recs[r + stride] : make_uint4(0, 0, 0, 0);
     const uint64_t row0 = ((uint64_t)rec.y << 32 | rec.x) * kSharedRows;
-    const uint32_t qbase = rec.z;
-    uint32_t pending = rec.w;
-    const uint64_t left = n_rows - row0;
-    const uint32_t tiles = left >= kSharedRows ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
-    uint32_t qw[QT];
-#pragma unroll
-    for (int t = 0; t < QT; t++) {
-      const uint32_t qi = qbase + 16 * t + rn;
-      qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
-    }
-    Expanded qa[QT];
-#pragma unroll
-    for (int t = 0; t < QT; t++) {
-      expand_word(qw[t], lut, qa[t]);
-      if (qbase + 16 * t + rn >= n_queries) qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;
-    }
-    const RowSource<kPlane> src(db, plane, row0, lane);
-    while (pending) {
-      const uint32_t g = (uint32_t)__builtin_ctz(pending);
-      pending &= pending - 1;
-      uint32_t w[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        // clamped into the sub-chunk's tiles (and, for raw rows, its rows)
-        const uint32_t tt = min(4 * g + (uint32_t)j, tiles - 1);
-        if (kPlane) {
-          w[j] = src(tt);
-        } else {
-          const uint64_t rr = min((uint64_t)tt * 16 + rn, left - 1);
-          w[j] = db[(row0 + rr) * 8 + kq];
-        }
-      }
-#pragma unroll 1
-      for (int j = 0; j < 4; j++) {
-        const uint32_t tt = 4 * g + (uint32_t)j;
-        if (tt >= tiles) break;  // the same for the whole wave
-        Expanded eg;
-        expand_word(j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3], lut, eg);
-        if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
-        // which query tiles have a pair at the bound (a wave-uniform mask)
-        uint32_t tmask = 0;
-#pragma unroll
-        for (int t = 0; t < QT; t++) {
-          const v4f c = mfma_prefix(qa[t], eg);
-          const int m = max(max(__float_as_int(c.x), __float_as_int(c.y)), max(__float_as_int(c.z), __float_as_int(c.w)));
-          tmask |= (wave_any(m >= ibound) ? 1u : 0u) << t;
-        }
-        // only those tiles' products are recomputed and checked, out of line:
-        // QT inlined checks keep several hundred registers live
-#pragma unroll
-        for (int t = 0; t < QT; t++)
-          if (tmask >> t & 1u)
-            stage_candidates(mfma_prefix(qa[t], eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
-                             n_queries, n_rows, threshold, stage, &n_stage, hits, capacity, counter, index_base);
-      }
-    }
+    resolve_record<QT, kPlane>(L, db, plane, n_rows, row0, rec.z, rec.w, queries, n_queries, threshold, hits,
+                               capacity, counter, index_base);
     rec = next;
   }
-  // the block's hits into the buffer: one run of slots
-  __syncthreads();
-  const uint32_t n = min(n_stage, kStageHits);
-  if (n == 0) return;  // the same for the whole block
-  if (threadIdx.x == 0) stage_base = atomicAdd(counter, (unsigned long long)n);
-  __syncthreads();
-  const unsigned long long base = stage_base;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-    if (base + i < capacity) hits[base + i] = stage[i];
+  resolve_flush(L, hits, capacity, counter);
+}
+
+// The shard holding MFMA row chunk `chunk` (views in chunk order).
+__device__ __forceinline__ uint32_t shard_of_chunk(const ShardView* __restrict__ shards, uint32_t n_shards,
+                                                   uint64_t chunk) {
+  uint32_t sh = 0;
+  while (sh + 1 < n_shards && shards[sh + 1].chunk_begin <= chunk) sh++;
+  return sh;
+}
+
+// The shared-rows layout over a set of shards, persistent: a fixed grid (a
+// multiple of 8 blocks) walks (global sub-chunk, group of kMfmaWaves query
+// groups) items, the sub-chunks of shard s numbered from its first chunk x
+// kSubsPerChunk.  The query count is read from device memory (`d_nq`, capped
+// at nq_cap), so one captured launch serves batches of any size.  Records
+// carry the global sub-chunk number.
+template <int QT, bool kPlane>
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_shards_shared_kernel(
+    const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
+    const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t threshold,
+    uint4* __restrict__ recs, uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
+  const uint32_t nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+  if (nq == 0) return;
+  __shared__ uint32_t lut[256];
+  __shared__ SharedRowsLds S;
+  init_lut(lut);
+  const uint32_t groups = (nq + QT * 16 - 1) / (QT * 16);
+  const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
+  const uint64_t items = total_chunks * kSubsPerChunk * sgroups;
+  for (uint64_t item = blockIdx.x; item < items; item += gridDim.x) {
+    uint64_t gsub;
+    uint32_t sgroup;
+    item_coords(item, sgroups, gsub, sgroup);
+    const ShardView& v = shards[shard_of_chunk(shards, n_shards, gsub / kSubsPerChunk)];
+    const uint64_t sub_base = v.chunk_begin * kSubsPerChunk;
+    mfma_item_shared<QT, kPlane>(lut, S, reinterpret_cast<const uint32_t*>(v.rows),
+                                 reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, gsub - sub_base, sgroup, queries,
+                                 nq, threshold, recs, rec_count, rec_cap, sub_base);
+  }
+}
+
+// Its recheck kernel: as pdq_match_resolve_kernel, with each record's shard
+// found from its global sub-chunk, and the per-wave rerun after an overflow
+// walking the items of every shard.
+template <int QT, bool kPlane>
+__global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_shards_kernel(
+    const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
+    const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t threshold,
+    MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter,
+    const uint4* __restrict__ recs, const uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
+  const uint32_t nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+  if (nq == 0) return;
+  __shared__ ResolveLds L;
+  resolve_init(L);
+  const int wave = (int)(threadIdx.x >> 6);
+  const uint32_t n_recs = *rec_count;
+  if (n_recs > rec_cap) {
+    constexpr int kHalves = kResolveWaves / kMfmaWaves;
+    const uint32_t groups = (nq + QT * 16 - 1) / (QT * 16);
+    const uint64_t items = total_chunks * groups;
+    for (uint64_t item = (uint64_t)blockIdx.x * kHalves + wave / kMfmaWaves; item < items;
+         item += (uint64_t)gridDim.x * kHalves) {
+      uint64_t chunk;
+      uint32_t group;
+      item_coords(item, groups, chunk, group);
+      const ShardView& v = shards[shard_of_chunk(shards, n_shards, chunk)];
+      mfma_item<QT, kPlane>(L.lut, reinterpret_cast<const uint32_t*>(v.rows),
+                            reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, chunk - v.chunk_begin, group, queries,
+                            nq, threshold, hits, capacity, counter, v.index_base, wave % kMfmaWaves);
+    }
+    return;
+  }
+  const uint32_t stride = gridDim.x * kResolveWaves;
+  uint32_t r = blockIdx.x * kResolveWaves + wave;
+  uint4 rec = r < n_recs ? recs[r] : make_uint4(0, 0, 0, 0);
+  for (; r < n_recs; r += stride) {
+    const uint4 next = r + stride < n_recs ? recs[r + stride] : make_uint4(0, 0, 0, 0);
+    const uint64_t gsub = (uint64_t)rec.y << 32 | rec.x;
+    const ShardView& v = shards[shard_of_chunk(shards, n_shards, gsub / kSubsPerChunk)];
+    const uint64_t row0 = (gsub - v.chunk_begin * kSubsPerChunk) * kSharedRows;
+    resolve_record<QT, kPlane>(L, reinterpret_cast<const uint32_t*>(v.rows),
+                               reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, row0, rec.z, rec.w, queries, nq,
+                               threshold, hits, capacity, counter, v.index_base);
+    rec = next;
+  }
+  resolve_flush(L, hits, capacity, counter);
 }
 
 // Persistent form over a set of database shards: a fixed grid (a multiple of
@@ -678,7 +794,18 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // call are enqueued together even when threads share a stream.  A stream
 // being captured into a graph cannot allocate: those calls run the per-wave
 // layout.
-constexpr uint32_t kRecordCap = 65536;  // 1 MiB: dense-hit batches overflow into a per-wave rerun
+constexpr uint32_t kRecordCap = 1u << 20;  // 16 MiB; more records (dense hits) rerun every item per wave
+
+// The record cap passed to the kernels: kRecordCap, or less when the
+// environment's YUME_PDQ_AMD_MATCH_RECORD_CAP (1 .. kRecordCap) asks for it,
+// read at each launch, so that tests reach the overflow path at small sizes.
+static uint32_t record_cap() {
+  const char* v = getenv("YUME_PDQ_AMD_MATCH_RECORD_CAP");
+  if (!v || !*v) return kRecordCap;
+  char* end = nullptr;
+  const unsigned long long c = strtoull(v, &end, 10);
+  return (end && *end == 0 && c >= 1 && c <= kRecordCap) ? (uint32_t)c : kRecordCap;
+}
 
 struct RecordArea {
   uint4* recs = nullptr;
@@ -736,6 +863,7 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
     hipError_t e = record_area(stream, &ra, &usable);
     if (e != hipSuccess) return e;
     if (usable) {
+      const uint32_t cap = record_cap();
       const uint64_t subs8 = ((n_rows + kSharedRows - 1) / kSharedRows + 7) / 8 * 8;
       const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
       const uint64_t blocks = subs8 * sgroups;
@@ -744,11 +872,11 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       if (plane)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
-                           ra.count, kRecordCap);
+                           ra.count, cap);
       else
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
-                           ra.count, kRecordCap);
+                           ra.count, cap);
       // one block per compute unit: plenty for the few records of sparse
       // hits, one hit-slot atomic per block, and the per-wave rerun after an
       // overflow walks all items with them
@@ -757,11 +885,11 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       if (plane)
         hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, true>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
-                           kRecordCap, chunks8, groups);
+                           cap, chunks8, groups);
       else
         hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, false>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
-                           kRecordCap, chunks8, groups);
+                           cap, chunks8, groups);
       return hipGetLastError();
     }
   }
@@ -834,6 +962,53 @@ hipError_t launch_match_mfma_shards(int qt, const ShardView* shards, uint32_t n_
                                        hits, capacity, counter, grid, stream);
     case 16: return launch_shards_qt<16>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, min_nq,
                                          threshold, hits, capacity, counter, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+uint64_t match_record_area_bytes() { return (uint64_t)kRecordCap * sizeof(uint4) + 256; }
+
+template <int QT>
+static hipError_t launch_shards_shared_qt(const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
+                                          bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                          uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                          unsigned long long* counter, void* area, uint32_t grid, uint32_t rgrid,
+                                          hipStream_t stream) {
+  uint4* recs = static_cast<uint4*>(area);
+  uint32_t* count = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(area) + (size_t)kRecordCap * sizeof(uint4));
+  const uint32_t* q = static_cast<const uint32_t*>(queries);
+  const dim3 block(64 * kMfmaWaves), rblock(64 * kResolveWaves);
+  const uint32_t cap = record_cap();
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, count);
+  if (plane) {
+    hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, true>), dim3(grid), block, 0, stream, shards, n_shards,
+                       total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    hipLaunchKernelGGL((pdq_match_resolve_shards_kernel<QT, true>), dim3(rgrid), rblock, 0, stream, shards, n_shards,
+                       total_chunks, q, d_nq, nq_cap, threshold, hits, capacity, counter, recs, count, cap);
+  } else {
+    hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, false>), dim3(grid), block, 0, stream, shards,
+                       n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    hipLaunchKernelGGL((pdq_match_resolve_shards_kernel<QT, false>), dim3(rgrid), rblock, 0, stream, shards, n_shards,
+                       total_chunks, q, d_nq, nq_cap, threshold, hits, capacity, counter, recs, count, cap);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_match_mfma_shards_shared(int qt, const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
+                                           bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
+                                           uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                           unsigned long long* counter, void* record_area, uint32_t grid,
+                                           uint32_t rgrid, hipStream_t stream) {
+  if (threshold >= 64 || grid == 0 || grid % 8 || rgrid == 0 || !record_area) return hipErrorInvalidValue;
+  switch (qt) {
+    case 2: return launch_shards_shared_qt<2>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, threshold,
+                                              hits, capacity, counter, record_area, grid, rgrid, stream);
+    case 4: return launch_shards_shared_qt<4>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, threshold,
+                                              hits, capacity, counter, record_area, grid, rgrid, stream);
+    case 8: return launch_shards_shared_qt<8>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap, threshold,
+                                              hits, capacity, counter, record_area, grid, rgrid, stream);
+    case 16: return launch_shards_shared_qt<16>(shards, n_shards, total_chunks, plane, queries, d_nq, nq_cap,
+                                                threshold, hits, capacity, counter, record_area, grid, rgrid, stream);
     default: return hipErrorInvalidValue;
   }
 }
