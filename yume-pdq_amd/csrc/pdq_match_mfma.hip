@@ -327,8 +327,9 @@ __device__ __forceinline__ void mfma_item(const uint32_t* __restrict__ lut, cons
 // Shared-rows work item, for batches of kMfmaWaves or more query groups: the
 // block's waves stream the same kSharedTiles 16-row tiles (rows
 // sub * kSharedRows ...), wave w against query group kMfmaWaves * sgroup + w.
-// For every group of 4 tiles, wave w loads and expands tile 4g + w once, into
-// LDS (double-buffered, one barrier per group), and every wave reads the 4
+// For every group of kGroupTiles tiles, wave w loads and expands tiles
+// kGroupTiles * g + kMfmaWaves * s + w (s < kStagePerWave) once, into LDS
+// (double-buffered, one barrier per group), and every wave reads the group's
 // expanded tiles with one 16-byte LDS read each.  Each wave would otherwise
 // load and expand every tile itself; shared, a product costs a quarter of the
 // loads and LUT reads, which is what more query tiles per wave would buy
@@ -337,12 +338,15 @@ __device__ __forceinline__ void mfma_item(const uint32_t* __restrict__ lut, cons
 // in mfma_item, reading its tiles' rows directly.
 constexpr int kSharedTiles = 64;
 constexpr uint64_t kSharedRows = (uint64_t)kSharedTiles * 16;
-static_assert(kSharedTiles % 4 == 0 && kSharedTiles / 4 <= 64, "one mask bit per group of 4 tiles");
+constexpr int kStagePerWave = 1;                          // tiles a wave stages per group
+constexpr int kGroupTiles = kMfmaWaves * kStagePerWave;  // tiles per barrier and per vote
+static_assert(kSharedTiles % kGroupTiles == 0 && kSharedTiles / kGroupTiles <= 32,
+              "one bit of a 32-bit record mask per group of tiles");
 static_assert((uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 % kSharedRows == 0, "sub-chunks of a chunk");
 constexpr uint64_t kSubsPerChunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 / kSharedRows;
 
 struct SharedRowsLds {
-  u32x4 ex[2][kMfmaWaves][64];  // two groups of kMfmaWaves expanded tiles, lane-major
+  u32x4 ex[2][kGroupTiles][64];  // two groups of expanded tiles, lane-major
 };
 
 // The shared-rows stream of one sub-chunk.  kFull (all kSharedTiles tiles
@@ -359,18 +363,20 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
   const int rn = lane & 15, kq = lane >> 4;
   const uint64_t left = n_rows - row0;
   const uint32_t tiles = kFull ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
-  const uint32_t ngroups = kFull ? (uint32_t)(kSharedTiles / 4) : (tiles + 3) / 4;
-  // This wave's word of tile 4g + wave, loaded unconditionally: the tile is
-  // clamped into the sub-chunk (the plane is padded to whole tiles) and a
-  // raw row past the end is clamped to the last row; stage() zeroes both.
-  auto word = [&](uint32_t g) -> uint32_t {
-    const uint32_t tt = min(4 * g + (uint32_t)wave, tiles - 1);
+  const uint32_t ngroups =
+      kFull ? (uint32_t)(kSharedTiles / kGroupTiles) : (tiles + kGroupTiles - 1) / kGroupTiles;
+  // This wave's word of its s-th tile of group g, loaded unconditionally: the
+  // tile is clamped into the sub-chunk (the plane is padded to whole tiles)
+  // and a raw row past the end is clamped to the last row; stage() zeroes
+  // both.
+  auto word = [&](uint32_t g, int s) -> uint32_t {
+    const uint32_t tt = min(kGroupTiles * g + kMfmaWaves * s + (uint32_t)wave, tiles - 1);
     if (kPlane || kFull) return src(tt);
     const uint64_t r = min((uint64_t)tt * 16 + rn, left - 1);
     return db[(row0 + r) * 8 + kq];
   };
-  auto stage = [&](uint32_t g, uint32_t w, int buf) {
-    const uint32_t tt = 4 * g + (uint32_t)wave;
+  auto stage = [&](uint32_t g, int s, uint32_t w, int buf) {
+    const uint32_t tt = kGroupTiles * g + kMfmaWaves * s + (uint32_t)wave;
     Expanded e;
     expand_word(w, lut, e);
     // zero operands (not the expansion of a zero word, which is all -1),
@@ -382,22 +388,25 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
     v.y = e.d[1] & keep;
     v.z = e.d[2] & keep;
     v.w = e.d[3] & keep;
-    S.ex[buf][wave][lane] = v;
+    S.ex[buf][kMfmaWaves * s + wave][lane] = v;
   };
-  // ring[a] holds this wave's word of group g0 + a + 1, loaded kRing groups
+  // ring[a] holds this wave's words of group g0 + a + 1, loaded kRing groups
   // ahead; with the inner loop unrolled the slots are fixed registers
   constexpr int kRing = 4;
-  static_assert((kSharedTiles / 4) % kRing == 0, "whole rounds of the ring");
-  uint32_t ring[kRing];
+  static_assert((kSharedTiles / kGroupTiles) % kRing == 0, "whole rounds of the ring");
+  uint32_t ring[kRing][kStagePerWave];
 #pragma unroll
-  for (int a = 0; a < kRing; a++) ring[a] = word(a + 1 < (int)ngroups ? a + 1 : ngroups - 1);
-  stage(0, word(0), 0);
+  for (int a = 0; a < kRing; a++)
+#pragma unroll
+    for (int st = 0; st < kStagePerWave; st++) ring[a][st] = word(a + 1 < (int)ngroups ? a + 1 : ngroups - 1, st);
+#pragma unroll
+  for (int st = 0; st < kStagePerWave; st++) stage(0, st, word(0, st), 0);
   __syncthreads();
   int mx[kMaxChains];
 #pragma unroll
   for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
   const int ibound = __float_as_int(bound);
-  uint64_t pending = 0;  // groups of 4 tiles whose products reached the bound
+  uint64_t pending = 0;  // groups of tiles whose products reached the bound
 #pragma unroll 1
   for (uint32_t g0 = 0; g0 < ngroups; g0 += kRing) {
 #pragma unroll
@@ -406,14 +415,17 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
       if (!kFull && g >= ngroups) break;  // the same for the whole block
       // The buffer written here was last read before the previous barrier.
       // After the last group this stages zeros that nobody reads.
-      if (kFull || g + 1 < ngroups) stage(g + 1, ring[a], (g + 1) & 1);
-      ring[a] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1);
-      // the group's 4 expanded tiles, read together so their latencies overlap
-      u32x4 vt[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) vt[j] = S.ex[g & 1][j][lane];
+      for (int st = 0; st < kStagePerWave; st++) {
+        if (kFull || g + 1 < ngroups) stage(g + 1, st, ring[a][st], (g + 1) & 1);
+        ring[a][st] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1, st);
+      }
+      // the group's expanded tiles, read together so their latencies overlap
+      u32x4 vt[kGroupTiles];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < kGroupTiles; j++) vt[j] = S.ex[g & 1][j][lane];
+#pragma unroll
+      for (int j = 0; j < kGroupTiles; j++) {
         Expanded rb;
         rb.d[0] = vt[j].x;
         rb.d[1] = vt[j].y;
@@ -450,7 +462,6 @@ __device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lu
                                                  uint32_t threshold, uint4* __restrict__ recs,
                                                  uint32_t* __restrict__ rec_count, uint32_t rec_cap,
                                                  uint64_t rec_sub_base) {
-  static_assert(kMfmaWaves == 4, "a group of 4 tiles, one per wave");
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: the tests on it are scalar
   const int rn = lane & 15, kq = lane >> 4;
@@ -526,7 +537,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 
 // The second kernel of the shared-rows layout, in blocks of kResolveWaves
 // waves.  Each wave takes records in turn: for every pending group of the
-// record's sub-chunk it recomputes the 4 tiles' products, and for each query
+// record's sub-chunk it recomputes the group's products, and for each query
 // tile with a pair at the bound checks every such pair with the exact
 // 256-bit distance.  The block's hits gather in LDS and take their slots in
 // the hit buffer with one atomic.  If the records overflowed their area (more
@@ -565,8 +576,8 @@ __device__ __forceinline__ void resolve_flush(ResolveLds& L, MatchHit* __restric
 
 // One record, checked by one wave: sub-chunk rows row0 ... of a database of
 // n_rows rows (global index index_base + row), queries qbase ...
-// (QT x 16), pending groups of 4 tiles.  Its loads are issued together (the
-// query words, then each pending group's 4 tile words).
+// (QT x 16), pending groups of kGroupTiles tiles.  Its loads are issued
+// together (the query words, then each pending group's tile words).
 template <int QT, bool kPlane>
 __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __restrict__ db,
                                                const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t row0,
@@ -596,11 +607,11 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __
   while (pending) {
     const uint32_t g = (uint32_t)__builtin_ctz(pending);
     pending &= pending - 1;
-    uint32_t w[4];
+    uint32_t w[kGroupTiles];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kGroupTiles; j++) {
       // clamped into the sub-chunk's tiles (and, for raw rows, its rows)
-      const uint32_t tt = min(4 * g + (uint32_t)j, tiles - 1);
+      const uint32_t tt = min(kGroupTiles * g + (uint32_t)j, tiles - 1);
       if (kPlane) {
         w[j] = src(tt);
       } else {
@@ -609,11 +620,14 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __
       }
     }
 #pragma unroll 1
-    for (int j = 0; j < 4; j++) {
-      const uint32_t tt = 4 * g + (uint32_t)j;
+    for (int j = 0; j < kGroupTiles; j++) {
+      const uint32_t tt = kGroupTiles * g + (uint32_t)j;
       if (tt >= tiles) break;  // the same for the whole wave
+      uint32_t wj = w[0];  // w[j], selected (a rolled loop cannot index registers)
+#pragma unroll
+      for (int k = 1; k < kGroupTiles; k++) wj = j == k ? w[k] : wj;
       Expanded eg;
-      expand_word(j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3], L.lut, eg);
+      expand_word(wj, L.lut, eg);
       if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
       // which query tiles have a pair at the bound (a wave-uniform mask)
       uint32_t tmask = 0;
