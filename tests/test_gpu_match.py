@@ -225,6 +225,29 @@ def test_overflow_reports_exact_count(gpu, oracle, dev):
     assert len(set(zip(hits["query"].tolist(), hits["index"].tolist()))) == 1000
 
 
+@pytest.mark.parametrize("nq", [64, 1024])
+def test_counter_reset_without_append(gpu, oracle, dev, nq):
+    """Without the append flag the matrix-core matcher clears the caller's
+    counter itself (in the kernel that clears its candidate records): a
+    counter holding garbage reads the exact count afterwards, also for an
+    empty database, and with the append flag it keeps adding.  64 queries
+    run the per-wave layout, 1024 the shared-rows one."""
+    import torch
+    db = synthetic.random_db(5003, seed=70)
+    q = synthetic.random_db(nq, seed=71)
+    synthetic.plant(db, q, 200, seed=72)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
+    exp_n, exp = oracle.match_all(db, q, 31)
+    count = torch.full((1,), 123456789, dtype=torch.int64, device=dev)
+    say(f"[match] counter reset nq={nq}")
+    total, hits = gpu.match(d_db, d_q, 31, count=count, path="mfma")
+    assert total == exp_n and np.array_equal(sort_triples(as_triples(hits)), exp)
+    total, _ = gpu.match(d_db, d_q, 31, count=count, path="mfma", append=True)
+    assert total == 2 * exp_n
+    empty = torch.empty((0, 32), dtype=torch.uint8, device=dev)
+    assert gpu.match(empty, d_q, 31, count=count, path="mfma")[0] == 0
+
+
 def test_index_base_and_append(gpu, oracle, dev):
     """Two shards into one hit buffer with global indices (sharded DB)."""
     import torch

@@ -345,22 +345,24 @@ static int match_impl(const uint8_t* d_db, const void* d_plane, uint64_t n_rows,
   if ((uintptr_t)d_plane % 16) return YUME_PDQ_AMD_EINVAL;
   if ((n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;
   if ((uintptr_t)d_db % 16 || (uintptr_t)d_queries % 4) return YUME_PDQ_AMD_EINVAL;
+  const bool force_valu = flags & YUME_PDQ_AMD_MATCH_VALU, force_mfma = flags & YUME_PDQ_AMD_MATCH_MFMA;
+  if (force_valu && force_mfma) return YUME_PDQ_AMD_EINVAL;
+  if (force_mfma && threshold >= 64) return YUME_PDQ_AMD_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (!(flags & YUME_PDQ_AMD_MATCH_APPEND)) {
+  static_assert(sizeof(YumePdqAmdHit) == sizeof(ypdq::MatchHit), "hit layout");
+  ypdq::MatchHit* hits = reinterpret_cast<ypdq::MatchHit*>(d_hits);
+  const bool reset = !(flags & YUME_PDQ_AMD_MATCH_APPEND);
+  const bool mfma = force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold));
+  // the matrix-core matcher clears the counter in its own first kernel
+  if (mfma)
+    return from_hip(ypdq::launch_match_mfma(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
+                                            d_count, index_base, st, reset));
+  if (reset) {
     // a kernel, not hipMemsetAsync, so the sequence can be graph-captured
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, st, d_count, 0ull);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return from_hip(e);
   }
-  static_assert(sizeof(YumePdqAmdHit) == sizeof(ypdq::MatchHit), "hit layout");
-  ypdq::MatchHit* hits = reinterpret_cast<ypdq::MatchHit*>(d_hits);
-  const bool force_valu = flags & YUME_PDQ_AMD_MATCH_VALU, force_mfma = flags & YUME_PDQ_AMD_MATCH_MFMA;
-  if (force_valu && force_mfma) return YUME_PDQ_AMD_EINVAL;
-  if (force_mfma && threshold >= 64) return YUME_PDQ_AMD_EINVAL;
-  const bool mfma = force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold));
-  if (mfma)
-    return from_hip(ypdq::launch_match_mfma(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
-                                            d_count, index_base, st));
   if (d_plane)
     return from_hip(ypdq::launch_match_plane(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
                                              d_count, index_base, st));

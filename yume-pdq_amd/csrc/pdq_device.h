@@ -141,10 +141,13 @@ hipError_t launch_match_plane(const void* db, const void* plane, uint64_t n_rows
 // Matrix-core prefix filter + exact re-check (threshold < 64, >= 32 queries).
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold);
 // plane: the database's prefix plane (launch_build_prefix_plane), or null to
-// read the raw rows.
+// read the raw rows.  reset_counter: zero *counter first, in the one-thread
+// kernel that clears the candidate-record count (one launch fewer than a
+// separate reset).
 hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                             unsigned long long* counter, uint64_t index_base, hipStream_t stream);
+                             unsigned long long* counter, uint64_t index_base, hipStream_t stream,
+                             bool reset_counter = false);
 uint64_t prefix_plane_bytes(uint64_t n_rows);
 hipError_t launch_build_prefix_plane(const void* db, uint64_t n_rows, void* plane, hipStream_t stream);
 hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,

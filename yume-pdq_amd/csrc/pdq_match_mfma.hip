@@ -856,12 +856,16 @@ static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable)
   return hipSuccess;
 }
 
-__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+// Clears the candidate-record count and, when asked, the hit counter.
+__global__ void reset_counts_kernel(uint32_t* rec_count, unsigned long long* counter) {
+  if (rec_count) *rec_count = 0u;
+  if (counter) *counter = 0ull;
+}
 
 template <int QT>
 static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                             uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                            unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
+                            unsigned long long* counter, uint64_t index_base, hipStream_t stream, bool reset) {
   const uint32_t groups = (n_queries + QT * 16 - 1) / (QT * 16);
   const uint32_t* d = static_cast<const uint32_t*>(db);
   const uint32_t* p = static_cast<const uint32_t*>(plane);
@@ -882,7 +886,7 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
       const uint64_t blocks = subs8 * sgroups;
       if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, ra.count);
+      hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, ra.count, reset ? counter : nullptr);
       if (plane)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
@@ -909,6 +913,7 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
   }
   const uint64_t blocks = chunks8 * groups;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  if (reset) hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, nullptr, counter);
   if (plane)
     hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, false>), dim3((unsigned)blocks), block, 0, stream, d, p,
                        n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, groups, nullptr, nullptr,
@@ -993,7 +998,7 @@ static hipError_t launch_shards_shared_qt(const ShardView* shards, uint32_t n_sh
   const uint32_t* q = static_cast<const uint32_t*>(queries);
   const dim3 block(64 * kMfmaWaves), rblock(64 * kResolveWaves);
   const uint32_t cap = record_cap();
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, count);
+  hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, count, nullptr);
   if (plane) {
     hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, true>), dim3(grid), block, 0, stream, shards, n_shards,
                        total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
@@ -1031,17 +1036,26 @@ bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold) { return thre
 
 hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* plane, uint64_t n_rows, const void* queries,
                                 uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                                unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
-  if (n_rows == 0 || n_queries == 0) return hipSuccess;
+                                unsigned long long* counter, uint64_t index_base, hipStream_t stream,
+                                bool reset_counter = false) {
   if (threshold >= 64) return hipErrorInvalidValue;
+  if (n_rows == 0 || n_queries == 0) {
+    if (!reset_counter) return hipSuccess;
+    hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, nullptr, counter);
+    return hipGetLastError();
+  }
   const uint32_t tiles = (n_queries + 15) / 16;
   if (tiles <= 2 || max_qt == 2)
-    return launch_qt<2>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<2>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
+                        reset_counter);
   if (tiles <= 4 || max_qt == 4)
-    return launch_qt<4>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<4>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
+                        reset_counter);
   if (tiles <= 8 || max_qt == 8)
-    return launch_qt<8>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
-  return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream);
+    return launch_qt<8>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
+                        reset_counter);
+  return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
+                       reset_counter);
 }
 
 #ifndef YPDQ_DEFAULT_QT
@@ -1051,11 +1065,12 @@ constexpr int kDefaultQt = YPDQ_DEFAULT_QT;
 
 hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                             unsigned long long* counter, uint64_t index_base, hipStream_t stream) {
+                             unsigned long long* counter, uint64_t index_base, hipStream_t stream,
+                             bool reset_counter) {
   // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
   // 1024 / 8192 queries); larger tiles cost occupancy
   return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
-                              index_base, stream);
+                              index_base, stream, reset_counter);
 }
 
 }  // namespace ypdq
