@@ -185,8 +185,8 @@ hipError_t ShardSet::launch_match(const MatcherChoice& c, const void* d_queries,
   if (shards_.empty()) return hipSuccess;
   const int cus = cus_;
   if (c.mfma && c.shared && record_area) {
-    // resident blocks of the streaming kernel as below; the recheck kernel
-    // runs one 8-wave block per CU
+    // resident blocks of the streaming kernel as below, and of the recheck
+    // kernel
     uint64_t grid = (uint64_t)cus * (c.qt >= 16 ? 2 : 4);
     const uint64_t groups = (nq_cap + c.qt * 16 - 1) / (c.qt * 16);
     const uint64_t items = total_chunks_ * 4 * ((groups + 3) / 4);
@@ -194,7 +194,7 @@ hipError_t ShardSet::launch_match(const MatcherChoice& c, const void* d_queries,
     grid = std::max<uint64_t>(8, grid / 8 * 8);
     return launch_match_mfma_shards_shared(c.qt, d_views_, (uint32_t)shards_.size(), total_chunks_, true, d_queries,
                                            d_nq, nq_cap, threshold, hits, capacity, counter, record_area,
-                                           (uint32_t)grid, (uint32_t)std::max(1, cus), stream);
+                                           (uint32_t)grid, match_resolve_blocks(cus), stream);
   }
   if (c.mfma) {
     // resident blocks: 4 waves per SIMD at 2..8 query tiles, 2 at 16

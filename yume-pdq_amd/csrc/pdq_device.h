@@ -119,6 +119,8 @@ hipError_t launch_match_mfma_shards(int qt, const ShardView* shards, uint32_t n_
 // record_area: match_record_area_bytes() of device memory used by no other
 // launch in flight (a batch context's own).
 uint64_t match_record_area_bytes();
+// Grid of the shared-rows recheck kernel on a GPU of `cus` compute units.
+uint32_t match_resolve_blocks(int cus);
 hipError_t launch_match_mfma_shards_shared(int qt, const ShardView* shards, uint32_t n_shards, uint64_t total_chunks,
                                            bool plane, const void* queries, const uint32_t* d_nq, uint32_t nq_cap,
                                            uint32_t threshold, MatchHit* hits, uint64_t capacity,

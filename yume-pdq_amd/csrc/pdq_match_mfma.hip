@@ -124,7 +124,7 @@ __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, u
 // launch.  Hits past it go straight to the buffer.
 constexpr uint32_t kStageHits = 4096;
 
-__device__ __noinline__ void stage_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
+__device__ __forceinline__ void stage_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
                                               float bound, const uint32_t* __restrict__ queries,
                                               const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
                                               uint32_t threshold, MatchHit* stage, uint32_t* n_stage,
@@ -545,6 +545,13 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // per-wave layout instead (each half of the block an item), which checks its
 // own candidates (the first kernel appended no hits), so the set stays exact.
 constexpr int kResolveWaves = 8;
+// Two blocks per CU (16 waves; the kernel fits 128 VGPRs at QT = 8): a
+// record's check is a chain of dependent loads, so waves in flight set its
+// rate.  Measured against one block per CU: 19 against 22.5 us at the
+// bench's sparse hits, 578 against 871 us at 271k hits per batch.
+constexpr int kResolveBlocksPerCu = 2;
+
+uint32_t match_resolve_blocks(int cus) { return (uint32_t)std::max(1, cus) * kResolveBlocksPerCu; }
 static_assert(kResolveWaves % kMfmaWaves == 0, "whole per-wave items per block");
 
 struct ResolveLds {
@@ -637,13 +644,20 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __
         const int m = max(max(__float_as_int(c.x), __float_as_int(c.y)), max(__float_as_int(c.z), __float_as_int(c.w)));
         tmask |= (wave_any(m >= ibound) ? 1u : 0u) << t;
       }
-      // only those tiles' products are recomputed and checked, out of line:
-      // QT inlined checks keep several hundred registers live
+      // only those tiles' products are recomputed and checked, by one
+      // inlined check in a loop over the mask: QT inlined copies keep
+      // several hundred registers live, and an out-of-line check saves and
+      // restores registers through scratch memory at every call
+      while (tmask) {
+        const int t = __builtin_ctz(tmask);
+        tmask &= tmask - 1;
+        Expanded a = qa[0];  // qa[t], selected (registers cannot be indexed)
 #pragma unroll
-      for (int t = 0; t < QT; t++)
-        if (tmask >> t & 1u)
-          stage_candidates(mfma_prefix(qa[t], eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
-                           n_queries, n_rows, threshold, L.stage, &L.n_stage, hits, capacity, counter, index_base);
+        for (int k = 1; k < QT; k++)
+          if (t == k) a = qa[k];
+        stage_candidates(mfma_prefix(a, eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
+                         n_queries, n_rows, threshold, L.stage, &L.n_stage, hits, capacity, counter, index_base);
+      }
     }
   }
 }
@@ -895,10 +909,9 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
                            ra.count, cap);
-      // one block per compute unit: plenty for the few records of sparse
-      // hits, one hit-slot atomic per block, and the per-wave rerun after an
-      // overflow walks all items with them
-      const uint64_t rgrid = std::min<uint64_t>((uint64_t)ra.cus, chunks8 * groups);
+      // the resident blocks of the recheck kernel; the per-wave rerun after
+      // an overflow walks all items with them
+      const uint64_t rgrid = std::min<uint64_t>(match_resolve_blocks(ra.cus), chunks8 * groups);
       const dim3 rblock(64 * kResolveWaves);
       if (plane)
         hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, true>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
