@@ -104,9 +104,9 @@ __device__ __forceinline__ void check_candidates(v4f cv, int t, uint64_t tbase, 
 
 // The same out of line, for the per-wave layout, whose rare path keeps the
 // stream's query operands (inlined there it spills registers into the tile
-// loop), and for the shared-rows layout's second kernel.  (The shared-rows
-// streaming kernel checks nothing itself: a call anywhere in it, even one
-// that never runs, cost its tile loop about 12 %, and inline checks 9 %.)
+// loop).  (The shared-rows streaming kernel checks nothing itself: a call
+// anywhere in it, even one that never runs, cost its tile loop about 12 %,
+// and inline checks 9 %; its second kernel checks with stage_candidates.)
 __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
                                                 float bound, const uint32_t* __restrict__ queries,
                                                 const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
@@ -125,11 +125,11 @@ __device__ __noinline__ void resolve_candidates(v4f cv, int t, uint64_t tbase, u
 constexpr uint32_t kStageHits = 4096;
 
 __device__ __forceinline__ void stage_candidates(v4f cv, int t, uint64_t tbase, uint32_t qbase, int rn, int kq,
-                                              float bound, const uint32_t* __restrict__ queries,
-                                              const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
-                                              uint32_t threshold, MatchHit* stage, uint32_t* n_stage,
-                                              MatchHit* __restrict__ hits, uint64_t capacity,
-                                              unsigned long long* __restrict__ counter, uint64_t index_base) {
+                                                float bound, const uint32_t* __restrict__ queries,
+                                                const uint32_t* __restrict__ db, uint32_t n_queries, uint64_t n_rows,
+                                                uint32_t threshold, MatchHit* stage, uint32_t* n_stage,
+                                                MatchHit* __restrict__ hits, uint64_t capacity,
+                                                unsigned long long* __restrict__ counter, uint64_t index_base) {
   const uint64_t row = tbase + rn;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
