@@ -44,10 +44,16 @@ enum {
   YUME_PDQ_AMD_INPUT_RGBA8 = 3
 };
 
-/* Dihedral sign algebra: EXACT negates every second column (the AVX-512
- * and scalar kernels, src/kernel/x86.rs:1026-1074, src/kernel/mod.rs:368-382);
- * AVX2 reproduces the reference's AVX2 kernel, whose column negation reaches
- * columns 0..7 only (src/kernel/x86.rs:330-376). */
+/* Dihedral sign algebra.  AVX2 reproduces the reference's x86 smart kernel,
+ * whose column negation reaches columns 0..7 only (src/kernel/x86.rs:330-376,
+ * visited by src/lib.rs:391-418): its FLOPPED, ROTATED_180,
+ * FLOPPED_ROTATED_270 and ROTATED_90 hashes are the reference's, not the true
+ * hashes of those orientations.  AVX2 is the default wherever a default
+ * exists (NULL options, the C++ ServerConfig, the Python binding).  EXACT
+ * negates every second column (the AVX-512 and scalar kernels,
+ * src/kernel/x86.rs:1026-1074, src/kernel/mod.rs:368-382): the hashes of the
+ * truly flipped and rotated compressed plane, an explicit opt-in.  Both cost
+ * the same. */
 enum { YUME_PDQ_AMD_DIHEDRAL_EXACT = 0, YUME_PDQ_AMD_DIHEDRAL_AVX2 = 1 };
 
 /* Order of the 8 hashes per image in every batched output. */

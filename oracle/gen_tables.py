@@ -17,7 +17,7 @@ crate, 128-bit precision, round-to-nearest):
 This script calls the system libmpfr through ctypes with the same operation
 order, so every intermediate is rounded exactly as the reference's were.  The
 product has its own generator (mpmath, `yume-pdq_amd/csrc/gen_tables.py`);
-`tests/test_oracle_tables.py` checks that both agree bit for bit and that they
+`tests/test_tables.py` checks that both agree bit for bit and that they
 match the closed forms.
 
 Usage: python oracle/gen_tables.py  (rewrites oracle/pdq_tables.h)

@@ -131,7 +131,7 @@ int main(int argc, char** argv) {
   // (k ~ U{0..40} distinct flips) are planted from a pool of 8192 hashes (the
   // dihedral hashes of 1024 images; past the 1024 computed here the pool is
   // random), so 1024 queries meet about 6,250 plants.
-  api_ok(yume_pdq_amd_hash_batch(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, n_images, YUME_PDQ_AMD_DIHEDRAL_EXACT,
+  api_ok(yume_pdq_amd_hash_batch(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, n_images, YUME_PDQ_AMD_DIHEDRAL_AVX2,
                                  d_hashes, d_q, d_t, nullptr, s),
          "hash_batch");
   hip_ok(hipStreamSynchronize(s), "sync");
@@ -194,7 +194,7 @@ int main(int argc, char** argv) {
                  }});
   all.push_back({"hash_u8", [&] {
                    api_ok(yume_pdq_amd_hash_batch(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, batch,
-                                                  YUME_PDQ_AMD_DIHEDRAL_EXACT, d_hashes, d_q, d_t, nullptr, s),
+                                                  YUME_PDQ_AMD_DIHEDRAL_AVX2, d_hashes, d_q, d_t, nullptr, s),
                           "hash_batch");
                  }});
   struct Variant {
@@ -210,7 +210,7 @@ int main(int argc, char** argv) {
                               {"hash_u8_f16x3", YUME_PDQ_AMD_DCT_F16X3, YUME_PDQ_AMD_LAYOUT_BLOCK8}};
   for (const Variant& v : variants) {
     all.push_back({v.name, [&, v] {
-                     YumePdqAmdHashOptions o = {YUME_PDQ_AMD_DIHEDRAL_EXACT, v.dct, v.layout, nullptr};
+                     YumePdqAmdHashOptions o = {YUME_PDQ_AMD_DIHEDRAL_AVX2, v.dct, v.layout, nullptr};
                      api_ok(yume_pdq_amd_hash_batch_ex(d_planes, YUME_PDQ_AMD_INPUT_U8, 512 * 512, batch, &o,
                                                        d_hashes, d_q, d_t, nullptr, nullptr, s),
                             "hash_batch_ex");

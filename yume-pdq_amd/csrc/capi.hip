@@ -271,7 +271,7 @@ int yume_pdq_amd_hash_batch(const void* d_planes, int input_kind, uint64_t plane
 int yume_pdq_amd_hash_batch_ex(const void* d_planes, int input_kind, uint64_t plane_stride_bytes, uint32_t n_images,
                                const YumePdqAmdHashOptions* opts, uint8_t* d_hashes, float* d_quality,
                                float* d_threshold, float* d_pdqf, float* d_buf1, void* stream) {
-  const YumePdqAmdHashOptions def = {YUME_PDQ_AMD_DIHEDRAL_EXACT, YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_AUTO,
+  const YumePdqAmdHashOptions def = {YUME_PDQ_AMD_DIHEDRAL_AVX2, YUME_PDQ_AMD_DCT_EXACT, YUME_PDQ_AMD_LAYOUT_AUTO,
                                      nullptr};
   const YumePdqAmdHashOptions& o = opts ? *opts : def;
   if (n_images == 0) return YUME_PDQ_AMD_OK;

@@ -136,7 +136,7 @@ class BatchContext {
   std::shared_ptr<ShardSet> shards_;
   uint32_t max_images_ = 0;
   int input_kind_ = kInputU8;
-  uint32_t mode_ = kDihedralExact;
+  uint32_t mode_ = kDihedralAvx2;
   uint32_t threshold_ = 31;
   uint64_t hit_capacity_ = 0;
   uint64_t plane_stride_ = 0;

@@ -24,7 +24,7 @@ struct ServerConfig {
   uint64_t hbm_budget_bytes = 0;  // device memory the batch contexts may hold
   uint32_t max_contexts = 4;      // batches in flight at most
   uint32_t threshold = 31;
-  uint32_t dihedral_mode = kDihedralExact;
+  uint32_t dihedral_mode = kDihedralAvx2;
   uint64_t hit_capacity = 1 << 20;  // per batch; a batch with more hits is re-matched into a larger buffer
   bool use_graph = true;
   bool busy_poll = true;  // spin (yielding) while batches are in flight instead of sleeping between polls

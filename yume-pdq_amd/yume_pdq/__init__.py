@@ -249,7 +249,7 @@ def _stream_ptr(stream):
     return C.c_void_p(stream.cuda_stream)
 
 
-def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=None, out=None, luma=None,
+def hash_batch(planes, dihedral_mode=DIHEDRAL_AVX2, want_pdqf=False, stream=None, out=None, luma=None,
                dct=DCT_EXACT, layout=LAYOUT_AUTO, want_buf1=False):
     """Hash n 512x512 planes resident on the GPU: torch f32 or u8 luma
     (n, 512, 512), or u8 RGB8 / RGBA8 pixels (n, 512, 512, 3 or 4) converted to
@@ -332,7 +332,7 @@ def hash_batch(planes, dihedral_mode=DIHEDRAL_EXACT, want_pdqf=False, stream=Non
     return res
 
 
-def dihedrals_batch(pdqf, thr_in, dihedral_mode=DIHEDRAL_EXACT, stream=None):
+def dihedrals_batch(pdqf, thr_in, dihedral_mode=DIHEDRAL_AVX2, stream=None):
     """visit_dihedrals on device PDQFs (n,16,16) with thresholds (n,): returns
     (hashes (n,8,32), quality (n,8), threshold (n,8), pdqfs (n,8,16,16))."""
     torch = _torch()
@@ -420,7 +420,7 @@ class HashMatch:
     .quality / .threshold (n, 8), .hits (capacity, 16) u8 records of
     HIT_DTYPE, .count (1,) int64 on the device."""
 
-    def __init__(self, planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_EXACT,
+    def __init__(self, planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_AVX2,
                  stream=None):
         torch = _torch()
         if not planes.is_cuda or planes.dim() != 3 or tuple(planes.shape[1:]) != (512, 512):
@@ -470,7 +470,7 @@ class HashMatch:
         return total, np.sort(arr, order=("index", "query"))
 
 
-def hash_match(planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_EXACT, stream=None):
+def hash_match(planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mode=DIHEDRAL_AVX2, stream=None):
     """One batch through yume_pdq_amd_hash_match_batch: returns (hashes (n,8,32),
     quality (n,8), threshold (n,8), total, hits sorted by (index, query))."""
     hm = HashMatch(planes, db, threshold, plane=plane, capacity=capacity, dihedral_mode=dihedral_mode,
@@ -719,7 +719,7 @@ class Batcher:
     """Batch context: hash up to max_images planes and match their 8n hashes
     against every shard of `shards`, replayed as one HIP graph per batch."""
 
-    def __init__(self, shards, max_images, input_kind=INPUT_U8, dihedral_mode=DIHEDRAL_EXACT, threshold=31,
+    def __init__(self, shards, max_images, input_kind=INPUT_U8, dihedral_mode=DIHEDRAL_AVX2, threshold=31,
                  hit_capacity=1 << 20):
         h = C.c_void_p()
         _check(lib().yume_pdq_amd_batcher_create(shards._h, int(max_images), int(input_kind), int(dihedral_mode),
@@ -800,7 +800,7 @@ class Server:
     hits (queries numbered 8 * image + dihedral within the request)."""
 
     def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
-                 dihedral_mode=DIHEDRAL_EXACT, hit_capacity=1 << 20, use_graph=True, busy_poll=True,
+                 dihedral_mode=DIHEDRAL_AVX2, hit_capacity=1 << 20, use_graph=True, busy_poll=True,
                  in_order_compute=True):
         cfg = ServerConfig(int(max_batch_images), int(hbm_budget_bytes), int(max_contexts), int(threshold),
                            int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0, 1 if busy_poll else 0,
