@@ -143,8 +143,8 @@ typedef struct YumePdqAmdHashOptions {
   const float *luma_coeff;  /* (R, G, B) for colour inputs, NULL = BT.601 */
 } YumePdqAmdHashOptions;
 
-/* yume_pdq_amd_hash_batch with options (NULL = the defaults of
- * yume_pdq_amd_hash_batch).  d_buf1: n_images x 128 x 128 floats of each
+/* yume_pdq_amd_hash_batch with options (NULL = the reference's AVX2
+ * dihedrals, the exact DCT, the automatic layout, BT.601).  d_buf1: n_images x 128 x 128 floats of each
  * compressed plane (jarosz_compress output with row and column 127 zero), or
  * NULL. */
 int yume_pdq_amd_hash_batch_ex(const void *d_planes, int input_kind, uint64_t plane_stride_bytes,
@@ -216,6 +216,17 @@ int yume_pdq_amd_match_with_plane(const uint8_t *d_db, const void *d_plane, uint
                                   const uint8_t *d_queries, uint32_t n_queries, uint32_t threshold,
                                   YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
                                   uint64_t index_base, uint32_t flags, void *stream);
+
+/*
+ * Lowers the number of candidate records the matrix-core matcher's
+ * shared-rows layout keeps per call (cap 1 .. 2^20; 0 restores 2^20, the
+ * initial value unless the environment's YUME_PDQ_AMD_MATCH_RECORD_CAP gives
+ * another).  Past the cap a call still returns the exact set: it rechecks
+ * every row chunk instead of its records.  For tests and measurement; takes
+ * effect at the next call (batch contexts: at their next graph capture).
+ * EINVAL for a cap above 2^20.  No reference counterpart.
+ */
+int yume_pdq_amd_match_set_record_cap(uint32_t cap);
 
 /*
  * Hash and match in one call: yume_pdq_amd_hash_batch of n_images planes, then

@@ -36,3 +36,12 @@ def gpu():
 def dev():
     import torch
     return torch.device("cuda:0")
+
+
+@pytest.fixture
+def record_cap_4096(gpu):
+    """The matrix-core matcher's candidate-record cap lowered to 4096 for one
+    test (its overflow path at small sizes), restored afterwards."""
+    gpu.set_match_record_cap(4096)
+    yield 4096
+    gpu.set_match_record_cap(0)

@@ -221,15 +221,14 @@ def test_batcher_shared_rows_planted(gpu, dev):
         s.close()
 
 
-def test_batcher_shared_rows_record_overflow(gpu, dev, monkeypatch):
+def test_batcher_shared_rows_record_overflow(gpu, dev, record_cap_4096):
     """256 images (2048 queries) at threshold 63 over 1M rows in two shards:
     every group of tiles is a candidate, 977 sub-chunks x 16 query groups,
-    more records than the 4096 allowed here (YUME_PDQ_AMD_MATCH_RECORD_CAP,
+    more records than the 4096 allowed here (set_match_record_cap,
     read when the graph is captured), so the captured recheck kernel matches
     every item again per wave.  The set equals the single-database
     vector-ALU matcher's."""
     import torch
-    monkeypatch.setenv("YUME_PDQ_AMD_MATCH_RECORD_CAP", "4096")
     imgs = torch.from_numpy(synthetic.images_u8(256)).to(dev)
     hq = gpu.hash_batch(imgs)[0].reshape(-1, 32).cpu().numpy()
     parts = [synthetic.random_db(n, seed=500 + k) for k, n in enumerate((600_000, 400_001))]

@@ -164,15 +164,14 @@ def test_mfma_every_pair_a_hit(gpu, oracle, dev, plane):
     assert set(map(tuple, t.tolist())) <= set(map(tuple, exp.tolist()))
 
 
-def test_mfma_candidate_records_overflow(gpu, dev, monkeypatch):
+def test_mfma_candidate_records_overflow(gpu, dev, record_cap_4096):
     """1,000,003 rows x 2048 queries at threshold 63: 977 sub-chunks x 16
     query groups, every one a candidate, are more candidate records than the
-    4096 the first pass may keep here (YUME_PDQ_AMD_MATCH_RECORD_CAP; the
-    default is 1,048,576), so the second pass matches every work item again
+    4096 the first pass may keep here (set_match_record_cap;
+    the default is 1,048,576), so the second pass matches every work item again
     with per-wave candidate checks.  The set equals the vector-ALU kernel's,
     and holds the planted pairs."""
     import torch
-    monkeypatch.setenv("YUME_PDQ_AMD_MATCH_RECORD_CAP", "4096")
     db = synthetic.random_db(1_000_003, seed=93)
     q = synthetic.random_db(2048, seed=94)
     synthetic.plant_fast(db, q, 20_000, seed=95)

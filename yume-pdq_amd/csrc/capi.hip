@@ -385,6 +385,10 @@ int yume_pdq_amd_match_with_plane(const uint8_t* d_db, const void* d_plane, uint
                     flags, stream);
 }
 
+int yume_pdq_amd_match_set_record_cap(uint32_t cap) {
+  return ypdq::set_match_record_cap(cap) ? YUME_PDQ_AMD_OK : YUME_PDQ_AMD_EINVAL;
+}
+
 int yume_pdq_amd_hash_match_batch(const void* d_planes, int input_kind, uint64_t plane_stride_bytes,
                                   uint32_t n_images, uint32_t dihedral_mode, const uint8_t* d_db,
                                   const void* d_plane, uint64_t n_rows, uint32_t threshold, uint8_t* d_hashes,

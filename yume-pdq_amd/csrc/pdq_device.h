@@ -150,6 +150,9 @@ hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                              unsigned long long* counter, uint64_t index_base, hipStream_t stream,
                              bool reset_counter = false);
+// Lowers the shared-rows layout's record cap (0 = the default, 1 .. 2^20);
+// false for a larger value.
+bool set_match_record_cap(uint32_t cap);
 uint64_t prefix_plane_bytes(uint64_t n_rows);
 hipError_t launch_build_prefix_plane(const void* db, uint64_t n_rows, void* plane, hipStream_t stream);
 hipError_t launch_match_first(const void* db, uint64_t n_rows, const void* queries, uint32_t n_queries,

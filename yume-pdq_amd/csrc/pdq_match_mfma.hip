@@ -23,6 +23,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -819,20 +820,31 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // Record areas of the shared-rows layout: one per (device, stream), made on
 // the stream's first such launch and kept for the life of the process, with
 // a mutex so that the count reset, the filter and the resolve kernel of one
-// call are enqueued together even when threads share a stream.  A stream
-// being captured into a graph cannot allocate: those calls run the per-wave
-// layout.
+// call are enqueued together even when threads share a stream.  At most
+// kMaxStreamAreas streams of a device get an area; calls on further streams,
+// and on a stream being captured into a graph (which cannot allocate), run
+// the per-wave layout, which needs none.
 constexpr uint32_t kRecordCap = 1u << 20;  // 16 MiB; more records (dense hits) rerun every item per wave
+constexpr size_t kMaxStreamAreas = 16;     // per device
 
-// The record cap passed to the kernels: kRecordCap, or less when the
-// environment's YUME_PDQ_AMD_MATCH_RECORD_CAP (1 .. kRecordCap) asks for it,
-// read at each launch, so that tests reach the overflow path at small sizes.
-static uint32_t record_cap() {
+// The record cap passed to the kernels: kRecordCap, or less when set by
+// set_match_record_cap (tests reach the overflow path at small sizes with
+// it); the environment's YUME_PDQ_AMD_MATCH_RECORD_CAP (1 .. kRecordCap) is
+// its initial value, read once.
+static uint32_t env_record_cap() {
   const char* v = getenv("YUME_PDQ_AMD_MATCH_RECORD_CAP");
   if (!v || !*v) return kRecordCap;
   char* end = nullptr;
   const unsigned long long c = strtoull(v, &end, 10);
   return (end && *end == 0 && c >= 1 && c <= kRecordCap) ? (uint32_t)c : kRecordCap;
+}
+static std::atomic<uint32_t> g_record_cap{env_record_cap()};
+static uint32_t record_cap() { return g_record_cap.load(std::memory_order_relaxed); }
+
+bool set_match_record_cap(uint32_t cap) {
+  if (cap > kRecordCap) return false;
+  g_record_cap.store(cap == 0 ? kRecordCap : cap, std::memory_order_relaxed);
+  return true;
 }
 
 struct RecordArea {
@@ -852,8 +864,10 @@ static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable)
   int dev = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   static std::map<std::pair<int, hipStream_t>, RecordArea> areas;  // guarded by g_record_mu
+  static std::map<int, size_t> per_device;                          // areas made per device
   auto it = areas.find({dev, stream});
   if (it == areas.end()) {
+    if (per_device[dev] >= kMaxStreamAreas) return hipSuccess;  // not usable: the per-wave layout
     void* p = nullptr;
     if ((e = hipMalloc(&p, (size_t)kRecordCap * sizeof(uint4) + 256)) != hipSuccess) return e;
     RecordArea ra;
@@ -864,6 +878,7 @@ static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable)
     ra.recs = static_cast<uint4*>(p);
     ra.count = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(p) + (size_t)kRecordCap * sizeof(uint4));
     it = areas.emplace(std::make_pair(dev, stream), ra).first;
+    per_device[dev]++;
   }
   *out = it->second;
   *usable = true;
@@ -879,7 +894,8 @@ __global__ void reset_counts_kernel(uint32_t* rec_count, unsigned long long* cou
 template <int QT>
 static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, const void* queries,
                             uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                            unsigned long long* counter, uint64_t index_base, hipStream_t stream, bool reset) {
+                            unsigned long long* counter, uint64_t index_base, hipStream_t stream, bool reset,
+                            const RecordArea* fixed) {
   const uint32_t groups = (n_queries + QT * 16 - 1) / (QT * 16);
   const uint32_t* d = static_cast<const uint32_t*>(db);
   const uint32_t* p = static_cast<const uint32_t*>(plane);
@@ -889,11 +905,17 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
   const uint64_t rows_per_chunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
   const uint64_t chunks8 = ((n_rows + rows_per_chunk - 1) / rows_per_chunk + 7) / 8 * 8;  // whole XCD rounds
   if (groups >= (uint32_t)kMfmaWaves) {
+    // the caller's fixed area, else this stream's own area
     std::lock_guard<std::mutex> g(g_record_mu);
     RecordArea ra;
     bool usable = false;
-    hipError_t e = record_area(stream, &ra, &usable);
-    if (e != hipSuccess) return e;
+    if (fixed) {
+      ra = *fixed;
+      usable = true;
+    } else {
+      hipError_t e = record_area(stream, &ra, &usable);
+      if (e != hipSuccess) return e;
+    }
     if (usable) {
       const uint32_t cap = record_cap();
       const uint64_t subs8 = ((n_rows + kSharedRows - 1) / kSharedRows + 7) / 8 * 8;
@@ -1047,28 +1069,39 @@ hipError_t launch_match_mfma_shards_shared(int qt, const ShardView* shards, uint
 
 bool match_mfma_applicable(uint32_t n_queries, uint32_t threshold) { return threshold < 64 && n_queries >= 32; }
 
-hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* plane, uint64_t n_rows, const void* queries,
-                                uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
-                                unsigned long long* counter, uint64_t index_base, hipStream_t stream,
-                                bool reset_counter = false) {
+// Query tiles per wave that launch_match_mfma_qt runs for n_queries.
+static int qt_for(int max_qt, uint32_t n_queries) {
+  const uint32_t tiles = (n_queries + 15) / 16;
+  if (tiles <= 2 || max_qt == 2) return 2;
+  if (tiles <= 4 || max_qt == 4) return 4;
+  if (tiles <= 8 || max_qt == 8) return 8;
+  return 16;
+}
+
+static hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* plane, uint64_t n_rows,
+                                       const void* queries, uint32_t n_queries, uint32_t threshold, MatchHit* hits,
+                                       uint64_t capacity, unsigned long long* counter, uint64_t index_base,
+                                       hipStream_t stream, bool reset_counter, const RecordArea* fixed) {
   if (threshold >= 64) return hipErrorInvalidValue;
   if (n_rows == 0 || n_queries == 0) {
     if (!reset_counter) return hipSuccess;
     hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, nullptr, counter);
     return hipGetLastError();
   }
-  const uint32_t tiles = (n_queries + 15) / 16;
-  if (tiles <= 2 || max_qt == 2)
-    return launch_qt<2>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
-                        reset_counter);
-  if (tiles <= 4 || max_qt == 4)
-    return launch_qt<4>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
-                        reset_counter);
-  if (tiles <= 8 || max_qt == 8)
-    return launch_qt<8>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
-                        reset_counter);
-  return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base, stream,
-                       reset_counter);
+  switch (qt_for(max_qt, n_queries)) {
+    case 2:
+      return launch_qt<2>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
+                          stream, reset_counter, fixed);
+    case 4:
+      return launch_qt<4>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
+                          stream, reset_counter, fixed);
+    case 8:
+      return launch_qt<8>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
+                          stream, reset_counter, fixed);
+    default:
+      return launch_qt<16>(db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter, index_base,
+                           stream, reset_counter, fixed);
+  }
 }
 
 #ifndef YPDQ_DEFAULT_QT
@@ -1083,7 +1116,7 @@ hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows,
   // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
   // 1024 / 8192 queries); larger tiles cost occupancy
   return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
-                              index_base, stream, reset_counter);
+                              index_base, stream, reset_counter, nullptr);
 }
 
 }  // namespace ypdq

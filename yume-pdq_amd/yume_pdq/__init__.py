@@ -126,6 +126,8 @@ def lib():
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
         L.yume_pdq_amd_match_with_plane.restype = i
         L.yume_pdq_amd_match_with_plane.argtypes = [vp, vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
+        L.yume_pdq_amd_match_set_record_cap.restype = i
+        L.yume_pdq_amd_match_set_record_cap.argtypes = [u32]
         L.yume_pdq_amd_hash_match_batch.restype = i
         L.yume_pdq_amd_hash_match_batch.argtypes = [vp, i, u64, u32, u32, vp, vp, u64, u32, vp, vp, vp, vp, u64,
                                                     vp, vp, vp]
@@ -478,6 +480,13 @@ def hash_match(planes, db, threshold, plane=None, capacity=1 << 16, dihedral_mod
     hm.run()
     total, arr = hm.results()
     return hm.hashes, hm.quality, hm.threshold, total, arr
+
+
+def set_match_record_cap(cap):
+    """Lower the matrix-core matcher's candidate-record cap (1 .. 2**20; 0 =
+    the default): calls past it still return the exact set, through the
+    per-chunk recheck.  For tests and measurement."""
+    _check(lib().yume_pdq_amd_match_set_record_cap(int(cap)), "match_set_record_cap")
 
 
 def match_uses_matrix_cores(n_queries, threshold, path=None):
