@@ -29,7 +29,8 @@ enum {
   YUME_PDQ_AMD_EOVERFLOW = -5, /* more hits than the buffer holds (count still exact) */
   YUME_PDQ_AMD_ETIMEOUT = -6,  /* a wait ran out of time */
   YUME_PDQ_AMD_EIO = -7,       /* a read or write of the stream failed */
-  YUME_PDQ_AMD_EEOF = -8       /* the input stream ended inside a frame */
+  YUME_PDQ_AMD_EEOF = -8,      /* the input stream ended inside a frame */
+  YUME_PDQ_AMD_ESCRATCH = -9   /* more scratch than yume_pdq_amd_match_noalloc holds */
 };
 
 /* Input planes: f32 luma (0..255), u8 luma, or 8-bit colour converted to luma
@@ -216,6 +217,30 @@ int yume_pdq_amd_match_with_plane(const uint8_t *d_db, const void *d_plane, uint
                                   const uint8_t *d_queries, uint32_t n_queries, uint32_t threshold,
                                   YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
                                   uint64_t index_base, uint32_t flags, void *stream);
+
+/*
+ * yume_pdq_amd_match_with_plane (d_plane given) or yume_pdq_amd_match
+ * (d_plane NULL) that allocates nothing, for callers that cannot call
+ * hipMalloc: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation.  No reference counterpart.  Same arguments, flags
+ * and results.  Where the matrix-core matcher keeps candidate records (batches
+ * of 385+ queries), they go to one static scratch area per GPU in the
+ * library's own device memory (2^20 records of 16 bytes + 256 bytes, held
+ * from the library's load until the process ends) instead of an area
+ * allocated per stream.  Limit: a call's worst case -- one record per
+ * 128-query group and 1024 database rows, i.e.
+ * ceil(ceil(n_rows / 1024) / 8) * 8 * ceil(n_queries / 512) * 4 records --
+ * must fit the area (at 1024 queries: up to 134M rows; at 10M rows: up to
+ * 13,312 queries); a larger call returns YUME_PDQ_AMD_ESCRATCH and enqueues
+ * nothing.  Every call on one GPU shares the area, which keeps the last
+ * call's records (the call clears the count it starts from): calls on
+ * different streams of one GPU that overlap in time race and must be
+ * serialised by the caller.
+ */
+int yume_pdq_amd_match_noalloc(const uint8_t *d_db, const void *d_plane, uint64_t n_rows,
+                               const uint8_t *d_queries, uint32_t n_queries, uint32_t threshold,
+                               YumePdqAmdHit *d_hits, uint64_t capacity, unsigned long long *d_count,
+                               uint64_t index_base, uint32_t flags, void *stream);
 
 /*
  * Lowers the number of candidate records the matrix-core matcher's

@@ -206,6 +206,8 @@ const char* yume_pdq_amd_strerror(int code) {
     case YUME_PDQ_AMD_ETIMEOUT: return "timed out";
     case YUME_PDQ_AMD_EIO: return "stream read or write failed";
     case YUME_PDQ_AMD_EEOF: return "input ended inside a frame";
+    case YUME_PDQ_AMD_ESCRATCH:
+      return "the call needs more candidate records than the no-allocation matcher's static scratch holds";
     default: return "unknown error";
   }
 }
@@ -340,7 +342,8 @@ int yume_pdq_amd_dihedrals_batch(const float* d_pdqf, const float* d_thr_in, uin
 
 static int match_impl(const uint8_t* d_db, const void* d_plane, uint64_t n_rows, const uint8_t* d_queries,
                       uint32_t n_queries, uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity,
-                      unsigned long long* d_count, uint64_t index_base, uint32_t flags, void* stream) {
+                      unsigned long long* d_count, uint64_t index_base, uint32_t flags, void* stream,
+                      bool noalloc = false) {
   if (!d_count || (capacity && !d_hits)) return YUME_PDQ_AMD_EINVAL;
   if ((uintptr_t)d_plane % 16) return YUME_PDQ_AMD_EINVAL;
   if ((n_rows && !d_db) || (n_queries && !d_queries)) return YUME_PDQ_AMD_EINVAL;
@@ -354,6 +357,12 @@ static int match_impl(const uint8_t* d_db, const void* d_plane, uint64_t n_rows,
   const bool reset = !(flags & YUME_PDQ_AMD_MATCH_APPEND);
   const bool mfma = force_mfma || (!force_valu && ypdq::match_mfma_applicable(n_queries, threshold));
   // the matrix-core matcher clears the counter in its own first kernel
+  if (mfma && noalloc) {
+    if (ypdq::match_mfma_scratch_records(n_rows, n_queries) > ypdq::match_scratch_records())
+      return YUME_PDQ_AMD_ESCRATCH;
+    return from_hip(ypdq::launch_match_mfma_noalloc(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits,
+                                                    capacity, d_count, index_base, st, reset));
+  }
   if (mfma)
     return from_hip(ypdq::launch_match_mfma(d_db, d_plane, n_rows, d_queries, n_queries, threshold, hits, capacity,
                                             d_count, index_base, st, reset));
@@ -383,6 +392,13 @@ int yume_pdq_amd_match_with_plane(const uint8_t* d_db, const void* d_plane, uint
   if (!d_plane && n_rows) return YUME_PDQ_AMD_EINVAL;
   return match_impl(d_db, d_plane, n_rows, d_queries, n_queries, threshold, d_hits, capacity, d_count, index_base,
                     flags, stream);
+}
+
+int yume_pdq_amd_match_noalloc(const uint8_t* d_db, const void* d_plane, uint64_t n_rows, const uint8_t* d_queries,
+                               uint32_t n_queries, uint32_t threshold, YumePdqAmdHit* d_hits, uint64_t capacity,
+                               unsigned long long* d_count, uint64_t index_base, uint32_t flags, void* stream) {
+  return match_impl(d_db, d_plane, n_rows, d_queries, n_queries, threshold, d_hits, capacity, d_count, index_base,
+                    flags, stream, true);
 }
 
 int yume_pdq_amd_match_set_record_cap(uint32_t cap) {

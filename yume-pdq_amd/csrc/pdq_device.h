@@ -150,6 +150,19 @@ hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                              unsigned long long* counter, uint64_t index_base, hipStream_t stream,
                              bool reset_counter = false);
+// The same matcher with the shared-rows layout's candidate records in the
+// library's static per-GPU scratch area instead of a per-stream allocation:
+// no allocation at all.  match_mfma_scratch_records gives the records a call
+// may write at most (0 when its layout keeps none); a call whose worst case
+// exceeds match_scratch_records() fails with hipErrorInvalidValue before
+// anything is enqueued.  Calls on one GPU share the area, so calls on
+// different streams must not overlap.
+uint32_t match_scratch_records();
+uint64_t match_mfma_scratch_records(uint64_t n_rows, uint32_t n_queries);
+hipError_t launch_match_mfma_noalloc(const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                                     uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                     unsigned long long* counter, uint64_t index_base, hipStream_t stream,
+                                     bool reset_counter);
 // Lowers the shared-rows layout's record cap (0 = the default, 1 .. 2^20);
 // false for a larger value.
 bool set_match_record_cap(uint32_t cap);

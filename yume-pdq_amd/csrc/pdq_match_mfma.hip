@@ -885,6 +885,44 @@ static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable)
   return hipSuccess;
 }
 
+// The record area of the no-allocation entry point: kScratchRecords records,
+// then the count, in the code object's own device memory.  Each GPU the
+// library runs on holds one copy (16 MiB + 256 B) until the process ends.
+// Host code never names the array in a launch: its device address, per
+// device, comes from hipGetSymbolAddress (scratch_area below).
+constexpr uint32_t kScratchRecords = 1u << 20;
+static __device__ __align__(256) uint4 scratch[kScratchRecords + 16];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices];  // guarded by g_scratch_mu
+static int g_scratch_cus[kMaxScratchDevices];
+
+uint32_t match_scratch_records() { return kScratchRecords; }
+
+// The current device's scratch area, its address looked up on the device's
+// first call (with that device current) and kept.
+static hipError_t scratch_area(RecordArea* out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    int cus = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    g_scratch_cus[dev] = cus;
+    g_scratch_ptr[dev] = p;
+  }
+  out->recs = static_cast<uint4*>(g_scratch_ptr[dev]);
+  out->count = reinterpret_cast<uint32_t*>(out->recs + kScratchRecords);
+  out->cus = g_scratch_cus[dev];
+  return hipSuccess;
+}
+
 // Clears the candidate-record count and, when asked, the hit counter.
 __global__ void reset_counts_kernel(uint32_t* rec_count, unsigned long long* counter) {
   if (rec_count) *rec_count = 0u;
@@ -905,7 +943,8 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
   const uint64_t rows_per_chunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16;
   const uint64_t chunks8 = ((n_rows + rows_per_chunk - 1) / rows_per_chunk + 7) / 8 * 8;  // whole XCD rounds
   if (groups >= (uint32_t)kMfmaWaves) {
-    // the caller's fixed area, else this stream's own area
+    // the caller's fixed area (the no-allocation entry point's scratch,
+    // sized for this call's worst case), else this stream's own area
     std::lock_guard<std::mutex> g(g_record_mu);
     RecordArea ra;
     bool usable = false;
@@ -917,7 +956,7 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       if (e != hipSuccess) return e;
     }
     if (usable) {
-      const uint32_t cap = record_cap();
+      const uint32_t cap = fixed ? kScratchRecords : record_cap();
       const uint64_t subs8 = ((n_rows + kSharedRows - 1) / kSharedRows + 7) / 8 * 8;
       const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
       const uint64_t blocks = subs8 * sgroups;
@@ -1117,6 +1156,27 @@ hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows,
   // 1024 / 8192 queries); larger tiles cost occupancy
   return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
                               index_base, stream, reset_counter, nullptr);
+}
+
+uint64_t match_mfma_scratch_records(uint64_t n_rows, uint32_t n_queries) {
+  const int qt = qt_for(kDefaultQt, n_queries);
+  const uint32_t groups = (n_queries + qt * 16 - 1) / (qt * 16);
+  if (n_rows == 0 || groups < (uint32_t)kMfmaWaves) return 0;  // the per-wave layout keeps no records
+  const uint64_t subs8 = ((n_rows + kSharedRows - 1) / kSharedRows + 7) / 8 * 8;
+  const uint64_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
+  return subs8 * sgroups * kMfmaWaves;  // at most one record per wave of each block
+}
+
+hipError_t launch_match_mfma_noalloc(const void* db, const void* plane, uint64_t n_rows, const void* queries,
+                                     uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
+                                     unsigned long long* counter, uint64_t index_base, hipStream_t stream,
+                                     bool reset_counter) {
+  if (match_mfma_scratch_records(n_rows, n_queries) > kScratchRecords) return hipErrorInvalidValue;
+  RecordArea ra;
+  hipError_t e = scratch_area(&ra);
+  if (e != hipSuccess) return e;
+  return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
+                              index_base, stream, reset_counter, &ra);
 }
 
 }  // namespace ypdq

@@ -126,6 +126,8 @@ def lib():
         L.yume_pdq_amd_match_first.argtypes = [vp, u64, vp, u32, u32, vp, vp]
         L.yume_pdq_amd_match_with_plane.restype = i
         L.yume_pdq_amd_match_with_plane.argtypes = [vp, vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
+        L.yume_pdq_amd_match_noalloc.restype = i
+        L.yume_pdq_amd_match_noalloc.argtypes = [vp, vp, u64, vp, u32, u32, vp, u64, vp, u64, u32, vp]
         L.yume_pdq_amd_match_set_record_cap.restype = i
         L.yume_pdq_amd_match_set_record_cap.argtypes = [u32]
         L.yume_pdq_amd_hash_match_batch.restype = i
@@ -371,11 +373,13 @@ def build_prefix_plane(db, stream=None):
 
 
 def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, hits=None,
-          count=None, append=False, sync=True, path=None, plane=None):
+          count=None, append=False, sync=True, path=None, plane=None, noalloc=False):
     """All (query, index, distance) with popcount(query ^ row) <= threshold.
 
     db: CUDA u8 tensor (n_rows, 32); queries: CUDA u8 tensor (n_queries, 32);
-    plane: the prefix plane of db (build_prefix_plane) or None.
+    plane: the prefix plane of db (build_prefix_plane) or None; noalloc: run
+    yume_pdq_amd_match_noalloc (the library allocates nothing; the caller
+    still passes hits and count to avoid torch allocations).
     Returns (total_count, hits ndarray of HIT_DTYPE sorted by (index, query))
     when sync, else the device (hits, count) tensors."""
     torch = _torch()
@@ -392,7 +396,12 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
         count = torch.zeros(1, dtype=torch.int64, device=dev)
     capacity = hits.shape[0]
     flags = (MATCH_APPEND if append else 0) | {None: 0, "valu": MATCH_VALU, "mfma": MATCH_MFMA}[path]
-    if plane is None:
+    if noalloc:
+        _check(lib().yume_pdq_amd_match_noalloc(
+            C.c_void_p(db.data_ptr()), C.c_void_p(plane.data_ptr() if plane is not None else 0), db.shape[0],
+            C.c_void_p(queries.data_ptr()), queries.shape[0], int(threshold), C.c_void_p(hits.data_ptr()),
+            capacity, C.c_void_p(count.data_ptr()), index_base, flags, _stream_ptr(stream)), "match_noalloc")
+    elif plane is None:
         _check(lib().yume_pdq_amd_match(
             C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
             int(threshold), C.c_void_p(hits.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
