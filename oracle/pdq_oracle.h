@@ -56,6 +56,10 @@ void oracle_hash_dihedrals(const float *in, int mode, uint8_t *hashes, float *qu
 uint32_t oracle_hamming(const uint8_t *a, const uint8_t *b);
 uint64_t oracle_match_all(const uint8_t *db, uint64_t n_rows, const uint8_t *queries,
                           uint32_t n_queries, uint32_t threshold, OracleHit *hits, uint64_t cap);
+/* oracle_match_all on n_threads threads (UINT64_MAX if a thread ran out of
+ * memory). */
+uint64_t oracle_match_all_mt(const uint8_t *db, uint64_t n_rows, const uint8_t *queries, uint32_t n_queries,
+                             uint32_t threshold, OracleHit *hits, uint64_t cap, uint32_t n_threads);
 int oracle_match_first(const uint8_t *db, uint64_t n_rows, const uint8_t *queries,
                        uint32_t n_queries, uint32_t threshold, uint64_t *row_out,
                        uint32_t *query_out);

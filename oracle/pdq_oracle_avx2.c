@@ -376,3 +376,87 @@ uint64_t oracle_cpu_baseline(const float *images, uint32_t n_pool, uint32_t n_im
   free(mj);
   return hits;
 }
+
+/* ---------------------------------------------------------------------------
+ * Brute-force exact match on n_threads threads: oracle_match_all's triples
+ * (every (query, row) with popcount(query ^ row) <= threshold, inclusive;
+ * row-major order, queries ascending within a row), computed over contiguous
+ * row ranges, one per thread, and concatenated in range order.  The checker
+ * for database sizes where the single-threaded loop takes minutes.  The
+ * first two words' distance is a lower bound of the full one, so rows past
+ * the threshold there are skipped -- the same predicate, evaluated exactly.
+ * Returns the total count; writes at most cap triples.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const uint8_t *db;
+  uint64_t first, last;
+  const uint64_t *q;
+  uint32_t nq, thr;
+  OracleHit *buf;
+  uint64_t n, cap;
+  int failed;
+} AllJob;
+
+static void *all_worker(void *arg) {
+  AllJob *j = (AllJob *)arg;
+  j->n = 0;
+  for (uint64_t r = j->first; r < j->last; r++) {
+    uint64_t x[4];
+    memcpy(x, j->db + r * 32, 32);
+    for (uint32_t k = 0; k < j->nq; k++) {
+      const uint64_t *y = j->q + 4 * (size_t)k;
+      uint32_t d = (uint32_t)(__builtin_popcountll(x[0] ^ y[0]) + __builtin_popcountll(x[1] ^ y[1]));
+      if (d > j->thr) continue;
+      d += (uint32_t)(__builtin_popcountll(x[2] ^ y[2]) + __builtin_popcountll(x[3] ^ y[3]));
+      if (d > j->thr) continue;
+      if (j->n == j->cap) {
+        uint64_t cap = j->cap ? 2 * j->cap : 4096;
+        OracleHit *nb = (OracleHit *)realloc(j->buf, cap * sizeof(OracleHit));
+        if (!nb) {
+          j->failed = 1;
+          return NULL;
+        }
+        j->buf = nb;
+        j->cap = cap;
+      }
+      j->buf[j->n].query = k;
+      j->buf[j->n].distance = d;
+      j->buf[j->n].index = r;
+      j->n++;
+    }
+  }
+  return NULL;
+}
+
+uint64_t oracle_match_all_mt(const uint8_t *db, uint64_t n_rows, const uint8_t *queries, uint32_t n_queries,
+                             uint32_t threshold, OracleHit *hits, uint64_t cap, uint32_t n_threads) {
+  if (n_threads < 1) n_threads = 1;
+  uint64_t *q = (uint64_t *)malloc((size_t)n_queries * 32 + 32);
+  memcpy(q, queries, (size_t)n_queries * 32);
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * n_threads);
+  AllJob *jobs = (AllJob *)calloc(n_threads, sizeof(AllJob));
+  for (uint32_t t = 0; t < n_threads; t++) {
+    jobs[t].db = db;
+    jobs[t].first = n_rows * t / n_threads;
+    jobs[t].last = n_rows * (t + 1) / n_threads;
+    jobs[t].q = q;
+    jobs[t].nq = n_queries;
+    jobs[t].thr = threshold;
+    pthread_create(&th[t], NULL, all_worker, &jobs[t]);
+  }
+  uint64_t total = 0;
+  int failed = 0;
+  for (uint32_t t = 0; t < n_threads; t++) {
+    pthread_join(th[t], NULL);
+    failed |= jobs[t].failed;
+  }
+  for (uint32_t t = 0; t < n_threads; t++) {
+    for (uint64_t i = 0; i < jobs[t].n; i++, total++)
+      if (total < cap) hits[total] = jobs[t].buf[i];
+    free(jobs[t].buf);
+  }
+  free(jobs);
+  free(th);
+  free(q);
+  return failed ? UINT64_MAX : total;
+}

@@ -60,6 +60,7 @@ def lib():
             "oracle_hamming": (u32, [vp, vp]),
             "oracle_match_all": (u64, [vp, u64, vp, u32, u32, vp, u64]),
             "oracle_match_first": (i, [vp, u64, vp, u32, u32, vp, vp]),
+            "oracle_match_all_mt": (u64, [vp, u64, vp, u32, u32, vp, u64, u32]),
             "oracle_cpu_has_vpopcntdq": (i, []),
             "oracle_cpu_match_count": (u64, [vp, u64, vp, u32, u32, i]),
             "oracle_cpu_baseline": (u64, [vp, u32, u32, vp, u64, u32, u32, i, vp, vp, vp]),
@@ -234,6 +235,30 @@ def match_all(db, queries, threshold, cap=None):
     arr = np.array([(hits[i].query, hits[i].index, hits[i].distance) for i in range(k)],
                    dtype=np.int64).reshape(-1, 3)
     return int(n), arr
+
+
+HIT_DTYPE = np.dtype([("query", "<u4"), ("distance", "<u4"), ("index", "<u8")])
+
+
+def default_threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def match_all_mt(db, queries, threshold, cap=1 << 24, n_threads=None):
+    """match_all on several threads (same triples, same row-major order):
+    returns (total, (k, 3) int64 array of (query, index, distance))."""
+    db = np.ascontiguousarray(db, np.uint8).reshape(-1, 32)
+    q = np.ascontiguousarray(queries, np.uint8).reshape(-1, 32)
+    buf = np.zeros(max(int(cap), 1), HIT_DTYPE)
+    n = lib().oracle_match_all_mt(_p(db), db.shape[0], _p(q), q.shape[0], int(threshold), _p(buf), int(cap),
+                                  int(n_threads or default_threads()))
+    if n == 0xFFFFFFFFFFFFFFFF:
+        raise MemoryError("oracle_match_all_mt: a worker ran out of memory")
+    k = min(int(n), int(cap))
+    b = buf[:k]
+    t = np.stack([b["query"].astype(np.int64), b["index"].astype(np.int64), b["distance"].astype(np.int64)], 1) \
+        if k else np.zeros((0, 3), np.int64)
+    return int(n), t
 
 
 def match_first(db, queries, threshold):

@@ -339,6 +339,25 @@ def test_matcher_brute_force_closed_form(oracle):
     assert oracle.match_all(db[:1], db[:1], 0)[0] == 1
 
 
+def test_threaded_matcher_equals_single_threaded(oracle):
+    """The threaded brute force gives oracle_match_all's triples in the same
+    row-major order, for any thread count, at thresholds where few, some and
+    all pairs match, and with a truncating cap (exact total)."""
+    from yume_pdq import synthetic
+    db = synthetic.random_db(30_011, seed=41)
+    q = synthetic.random_db(40, seed=42)
+    synthetic.plant(db, q, 500, seed=43)
+    for thr in (0, 31, 40, 256):
+        n, ref = oracle.match_all(db[:3001] if thr == 256 else db, q, thr, cap=200_000)
+        for threads in (1, 3, 8):
+            m, got = oracle.match_all_mt(db[:3001] if thr == 256 else db, q, thr, n_threads=threads)
+            assert m == n and np.array_equal(got, ref), (thr, threads)
+    n, ref = oracle.match_all(db, q, 40)
+    m, got = oracle.match_all_mt(db, q, 40, cap=100, n_threads=4)
+    assert m == n > 100 and np.array_equal(got, ref[:100])
+    assert oracle.match_all_mt(db[:0], q, 31)[0] == 0
+
+
 # --- precise (reference) kernel and the FP64 oracle -------------------------------------
 
 def _tent_closed_form_f64(img, centres, w):
