@@ -107,8 +107,16 @@ int main(int argc, char** argv) {
     uint64_t* w = reinterpret_cast<uint64_t*>(db.data());
     for (uint64_t i = 0; i < rows * 4; i++) w[i] = rng();
   }
+  // 256 distinct planes; larger batches repeat them (the kernels' cost does
+  // not depend on the pixels)
   std::vector<uint8_t> planes((size_t)n_images * 512 * 512);
-  for (uint32_t k = 0; k < n_images; k++) make_plane(k, planes.data() + (size_t)k * 512 * 512);
+  for (uint32_t k = 0; k < n_images; k++) {
+    uint8_t* dst = planes.data() + (size_t)k * 512 * 512;
+    if (k < 256)
+      make_plane(k, dst);
+    else
+      std::memcpy(dst, planes.data() + (size_t)(k % 256) * 512 * 512, 512 * 512);
+  }
 
   hipStream_t s;
   hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");

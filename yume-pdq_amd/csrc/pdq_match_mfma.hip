@@ -180,6 +180,77 @@ __device__ __forceinline__ void tile_max(const Expanded (&qa)[QT], const Expande
   }
 }
 
+// Packed candidate flags: the filter of the streaming loops.  The products
+// of one row tile with two query tiles share one accumulator.  The first
+// MFMA adds dot_a to C0 = 2^23 + K + 512 K, the second adds 512 dot_b (its
+// query operand scaled by 2^9 through the MFMA's block scale), so each
+// result is
+//     2^23 + f_a + 512 f_b,   f = dot + K,   K = 256 - bound,
+// an integer in [2^23, 2^24) -- as is every partial sum, in any order of
+// accumulation -- so it is exact, and its low 23 bits are the two fields.
+// dot is in [-128, 128], so f is in [128 - bound, 384 - bound], below 512
+// (no carry between fields; bound = 128 - 2t is in [2, 128] for t < 64), and
+// f >= 256 exactly when dot >= bound: bit 8 flags pair a, bit 17 pair b.
+// One OR of the four results' bits then tests 8 pairs, where a maximum
+// needs one instruction per 2 values.
+constexpr uint32_t kFlagMask = (1u << 8) | (1u << 17);
+constexpr int kFlagChains = 4;
+constexpr int kScale1 = 127;      // 2^0
+constexpr int kScale512 = 127 + 9;  // 2^9
+
+__device__ __forceinline__ v4f flags_c0(float bound) {
+  const float k = 256.0f - bound;
+  const float c = 8388608.0f + k + 512.0f * k;  // exact: an integer below 2^24
+  return v4f{c, c, c, c};
+}
+
+template <int kScaleA>
+__device__ __forceinline__ v4f mfma_acc(const Expanded& a, const Expanded& b, const v4f& c) {
+  v8i av = {(int)a.d[0], (int)a.d[1], (int)a.d[2], (int)a.d[3], 0, 0, 0, 0};
+  v8i bv = {(int)b.d[0], (int)b.d[1], (int)b.d[2], (int)b.d[3], 0, 0, 0, 0};
+  return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv, c, 4, 4, 0, kScaleA, 0, kScale1);
+}
+
+__device__ __forceinline__ void fold_flags(const v4f& v, uint32_t& m) {
+  m = m | __float_as_uint(v.x) | __float_as_uint(v.y);
+  m = m | __float_as_uint(v.z) | __float_as_uint(v.w);
+}
+
+// Pair P (row tile P / (QT/2), query tiles 2 (P % (QT/2)) and +1): its first
+// product issues at step P into result register P % 4, its second at step
+// P + 1 (one product after, so the accumulation does not wait), and its bits
+// are folded at step P + 4, before the register takes pair P + 4.  The four
+// result registers are named, not an array, so the compiler keeps each in
+// place.
+template <int QT, int kTiles, int S>
+__device__ __forceinline__ void group_flags_step(const Expanded (&qa)[QT], const Expanded (&rb)[kTiles],
+                                                 const v4f& c0, uint32_t (&acc)[kFlagChains], v4f& r0, v4f& r1,
+                                                 v4f& r2, v4f& r3) {
+  constexpr int kHalf = QT / 2;
+  constexpr int kN = kHalf * kTiles;  // pairs
+  v4f& cur = (S % 4 == 0) ? r0 : (S % 4 == 1) ? r1 : (S % 4 == 2) ? r2 : r3;
+  v4f& prev = (S % 4 == 1) ? r0 : (S % 4 == 2) ? r1 : (S % 4 == 3) ? r2 : r3;
+  if constexpr (S >= 4) fold_flags(cur, acc[(S - 4) % kFlagChains]);
+  if constexpr (S < kN) cur = mfma_acc<kScale1>(qa[2 * (S % kHalf)], rb[S / kHalf], c0);
+  if constexpr (S >= 1 && S - 1 < kN)
+    prev = mfma_acc<kScale512>(qa[2 * ((S - 1) % kHalf) + 1], rb[(S - 1) / kHalf], prev);
+  // keep this order: hoisting the products runs out of registers
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (S + 1 < kN + 4) group_flags_step<QT, kTiles, S + 1>(qa, rb, c0, acc, r0, r1, r2, r3);
+}
+
+template <int QT, int kTiles>
+__device__ __forceinline__ void group_flags(const Expanded (&qa)[QT], const Expanded (&rb)[kTiles], const v4f& c0,
+                                            uint32_t (&acc)[kFlagChains]) {
+  static_assert(QT % 2 == 0 && (QT / 2) * kTiles >= 4, "pairs of query tiles, four pairs in flight");
+  v4f r0, r1, r2, r3;
+  group_flags_step<QT, kTiles, 0>(qa, rb, c0, acc, r0, r1, r2, r3);
+}
+
+__device__ __forceinline__ bool flags_set(const uint32_t (&acc)[kFlagChains]) {
+  return ((acc[0] | acc[1] | acc[2] | acc[3]) & kFlagMask) != 0;
+}
+
 // The rare path: recompute one tile's products and re-check every pair that
 // clears the bound with the exact 256-bit distance.
 template <int QT>
@@ -239,9 +310,13 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
 #pragma unroll
   for (int a = 0; a < kAhead; a++) ring[a] = fetch(a < (int)tiles ? a : tiles - 1);
   int mx[kMaxChains];
+  uint32_t acc[kFlagChains];
 #pragma unroll
   for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
+#pragma unroll
+  for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
   const int ibound = __float_as_int(bound);
+  const v4f c0 = flags_c0(bound);
   uint64_t pending = 0;  // groups whose products reached the bound
 #pragma unroll 1
   for (uint32_t tile = 0; tile < tiles; tile += kAhead) {
@@ -256,12 +331,19 @@ __device__ __forceinline__ void stream_tiles(const Expanded (&qa)[QT], const uin
       // and drop it): a load under a branch makes the wait counters unknown
       // to the compiler, which then waits for every load in flight.
       ring[a] = fetch(t + kAhead < tiles ? t + kAhead : tiles - 1);
-      tile_max<QT>(qa, rb, mx);
+      if constexpr (QT >= 8) {
+        const Expanded rbs[1] = {rb};
+        group_flags<QT, 1>(qa, rbs, c0, acc);
+      } else {
+        tile_max<QT>(qa, rb, mx);
+      }
       if (a % kGroup == kGroup - 1) {
         const int mall = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
-        pending |= (uint64_t)(wave_any(mall >= ibound) ? 1u : 0u) << (t / kGroup);
+        pending |= (uint64_t)(wave_any(mall >= ibound || flags_set(acc)) ? 1u : 0u) << (t / kGroup);
 #pragma unroll
         for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
+#pragma unroll
+        for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
       }
     }
   }
@@ -403,10 +485,10 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
 #pragma unroll
   for (int st = 0; st < kStagePerWave; st++) stage(0, st, word(0, st), 0);
   __syncthreads();
-  int mx[kMaxChains];
+  uint32_t acc[kFlagChains];
 #pragma unroll
-  for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
-  const int ibound = __float_as_int(bound);
+  for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
+  const v4f c0 = flags_c0(bound);
   uint64_t pending = 0;  // groups of tiles whose products reached the bound
 #pragma unroll 1
   for (uint32_t g0 = 0; g0 < ngroups; g0 += kRing) {
@@ -425,22 +507,21 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
       u32x4 vt[kGroupTiles];
 #pragma unroll
       for (int j = 0; j < kGroupTiles; j++) vt[j] = S.ex[g & 1][j][lane];
+      Expanded rb[kGroupTiles];
 #pragma unroll
       for (int j = 0; j < kGroupTiles; j++) {
-        Expanded rb;
-        rb.d[0] = vt[j].x;
-        rb.d[1] = vt[j].y;
-        rb.d[2] = vt[j].z;
-        rb.d[3] = vt[j].w;
-        tile_max<QT>(qa, rb, mx);
+        rb[j].d[0] = vt[j].x;
+        rb[j].d[1] = vt[j].y;
+        rb[j].d[2] = vt[j].z;
+        rb[j].d[3] = vt[j].w;
       }
-      const int mall = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
+      group_flags<QT, kGroupTiles>(qa, rb, c0, acc);
       // the vote is a wave-uniform mask, so the pending bits stay in scalar
       // registers
-      const uint64_t vote = __ballot(mall >= ibound);
+      const uint64_t vote = __ballot(flags_set(acc));
       pending |= (uint64_t)(vote != 0) << g;
 #pragma unroll
-      for (int m = 0; m < kMaxChains; m++) mx[m] = INT32_MIN;
+      for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
       __syncthreads();
     }
   }
@@ -516,7 +597,7 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
 // (4+ query groups), 1024-row sub-chunks x groups of kMfmaWaves query groups,
 // whose candidates become records for pdq_match_resolve_kernel.
 template <int QT, bool kPlane, bool kShared>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
     uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups,
@@ -716,7 +797,7 @@ __device__ __forceinline__ uint32_t shard_of_chunk(const ShardView* __restrict__
 // at nq_cap), so one captured launch serves batches of any size.  Records
 // carry the global sub-chunk number.
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_shards_shared_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_shards_shared_kernel(
     const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
     const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t threshold,
     uint4* __restrict__ recs, uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
@@ -794,7 +875,7 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_shards_k
 // serves batches of any size; batches of fewer than min_nq queries are left
 // to the vector-ALU kernel and this one returns at once.
 template <int QT, bool kPlane>
-__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 5))) pdq_match_mfma_shards_kernel(
+__global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_shards_kernel(
     const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
     const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t min_nq,
     uint32_t threshold, MatchHit* __restrict__ hits, uint64_t capacity, unsigned long long* __restrict__ counter) {
