@@ -5,7 +5,8 @@
 //                   scale for the FETCH_SIZE / WRITE_SIZE counters)
 //   match_x8        exhaustive match of one image's 8 dihedral hashes
 //   match_x1024     exhaustive match of 128 images' 1024 dihedral hashes
-//   match_plane_x8, match_plane_x1024  the same over the database's prefix plane
+//   match_plane_x8, match_plane_x256, match_plane_x1024  the same over the
+//                   database's prefix plane (256 queries: the per-wave layout)
 //   build_plane     building the prefix plane (not in the default set)
 //   hash_u8         fused hash of --batch u8 planes (automatic layout)
 //   hash_u8_wave, hash_u8_block4, hash_u8_block8   the same in one layout
@@ -193,6 +194,11 @@ int main(int argc, char** argv) {
   all.push_back({"match_plane_x8", [&] {
                    api_ok(yume_pdq_amd_match_with_plane(d_db, d_plane, rows, d_hashes, 8, 31, d_hits, cap, d_count, 0,
                                                         0, s),
+                          "match_with_plane");
+                 }});
+  all.push_back({"match_plane_x256", [&] {
+                   api_ok(yume_pdq_amd_match_with_plane(d_db, d_plane, rows, d_hashes, 256, 31, d_hits, cap, d_count,
+                                                        0, 0, s),
                           "match_with_plane");
                  }});
   all.push_back({"match_plane_x1024", [&] {

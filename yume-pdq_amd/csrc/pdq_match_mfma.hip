@@ -211,9 +211,13 @@ __device__ __forceinline__ v4f mfma_acc(const Expanded& a, const Expanded& b, co
   return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv, c, 4, 4, 0, kScaleA, 0, kScale1);
 }
 
+// The empty asm pins m here: without it the compiler gathers every fold of a
+// group into one OR tree at the group's end, which keeps all the group's
+// results live at once.
 __device__ __forceinline__ void fold_flags(const v4f& v, uint32_t& m) {
   m = m | __float_as_uint(v.x) | __float_as_uint(v.y);
   m = m | __float_as_uint(v.z) | __float_as_uint(v.w);
+  asm volatile("" : "+v"(m));
 }
 
 // Pair P (row tile P / (QT/2), query tiles 2 (P % (QT/2)) and +1): its first
