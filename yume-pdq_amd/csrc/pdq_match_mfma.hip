@@ -462,10 +462,10 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
     const uint64_t r = min((uint64_t)tt * 16 + rn, left - 1);
     return db[(row0 + r) * 8 + kq];
   };
-  auto stage = [&](uint32_t g, int s, uint32_t w, int buf) {
+  // staging in two halves: the table reads (expand_word), then the store,
+  // with the group's products between them
+  auto store = [&](uint32_t g, int s, const Expanded& e, int buf) {
     const uint32_t tt = kGroupTiles * g + kMfmaWaves * s + (uint32_t)wave;
-    Expanded e;
-    expand_word(w, lut, e);
     // zero operands (not the expansion of a zero word, which is all -1),
     // selected without a branch
     const bool have = kFull ? tt < tiles : (tt < tiles && (uint64_t)tt * 16 + rn < left);
@@ -476,6 +476,11 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
     v.z = e.d[2] & keep;
     v.w = e.d[3] & keep;
     S.ex[buf][kMfmaWaves * s + wave][lane] = v;
+  };
+  auto stage = [&](uint32_t g, int s, uint32_t w, int buf) {
+    Expanded e;
+    expand_word(w, lut, e);
+    store(g, s, e, buf);
   };
   // ring[a] holds this wave's words of group g0 + a + 1, loaded kRing groups
   // ahead; with the inner loop unrolled the slots are fixed registers
@@ -500,17 +505,22 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
     for (int a = 0; a < kRing; a++) {
       const uint32_t g = g0 + a;
       if (!kFull && g >= ngroups) break;  // the same for the whole block
-      // The buffer written here was last read before the previous barrier.
-      // After the last group this stages zeros that nobody reads.
-#pragma unroll
-      for (int st = 0; st < kStagePerWave; st++) {
-        if (kFull || g + 1 < ngroups) stage(g + 1, st, ring[a][st], (g + 1) & 1);
-        ring[a][st] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1, st);
-      }
-      // the group's expanded tiles, read together so their latencies overlap
+      // The group's expanded tiles, read together so their latencies
+      // overlap, and before the staging below: LDS operations complete in
+      // order, so the products then wait for these reads only, not for the
+      // table reads and the write of the next group.
       u32x4 vt[kGroupTiles];
 #pragma unroll
       for (int j = 0; j < kGroupTiles; j++) vt[j] = S.ex[g & 1][j][lane];
+      __builtin_amdgcn_sched_barrier(0);
+      // The next group's table reads are issued now and its store follows
+      // the products, so neither waits for the other.  The buffer written
+      // here was last read before the previous barrier; after the last group
+      // this stages rows nobody reads.
+      Expanded next[kStagePerWave];
+#pragma unroll
+      for (int st = 0; st < kStagePerWave; st++) expand_word(ring[a][st], lut, next[st]);
+      __builtin_amdgcn_sched_barrier(0);
       Expanded rb[kGroupTiles];
 #pragma unroll
       for (int j = 0; j < kGroupTiles; j++) {
@@ -520,6 +530,12 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
         rb[j].d[3] = vt[j].w;
       }
       group_flags<QT, kGroupTiles>(qa, rb, c0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int st = 0; st < kStagePerWave; st++) {
+        if (kFull || g + 1 < ngroups) store(g + 1, st, next[st], (g + 1) & 1);
+        ring[a][st] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1, st);
+      }
       // the vote is a wave-uniform mask, so the pending bits stay in scalar
       // registers
       const uint64_t vote = __ballot(flags_set(acc));
