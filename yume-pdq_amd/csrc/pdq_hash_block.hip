@@ -392,17 +392,24 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
   // C. stage 2 and the tail
   if (kPrec == kDctExact || literal) {
     if (wv < 4) stage2_column(d2, W, lane, wv);
-    __syncthreads();
-    if (wv != 0) return;
-    write_tmp(W, lane, out, img);
-  } else {
-    if (wv != 0) return;
-    write_tmp(W, lane, out, img);
+  } else if (wv == 0) {
     typename Precision<kPrec>::Op d[4];
     dct_operands<kPrec>(lane, d);
     stage2_mfma<kPrec>(W, lane, d);
   }
-  dihedral_tail(W, lane, mode, out, img, nullptr);
+  __syncthreads();
+  if (wv == 0) write_tmp(W, lane, out, img);
+  // the eight orientations' PDQFs, written by every wave (two orientations
+  // per element and thread with eight waves)
+  {
+    constexpr int kPer = 8 / (kWaves / 4);  // orientations per thread
+    const int idx = t & 255, d0 = (t >> 8) * kPer;
+    dihedral_pdqf_element(W, idx, d0, kPer, mode);
+  }
+  __syncthreads();
+  // the quality on wave 1 while wave 0 searches the thresholds
+  if (wv == 1) tail_quality(W, lane, out, img, true);
+  if (wv == 0) tail_thresholds_bits(W, lane, out, img, nullptr);
 }
 
 template <typename T, int kPrec, int kWaves>

@@ -438,6 +438,86 @@ __device__ __forceinline__ void median_search8(const float* __restrict__ P, int 
   bit_threshold = last;
 }
 
+// The value of lane l ^ 32 (v_permlane32_swap: the upper half of the first
+// operand trades places with the lower half of the second).
+__device__ __forceinline__ uint32_t half_swap(uint32_t x, int lane) {
+  const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+  return lane < 32 ? r[1] : r[0];
+}
+
+// median_search8 for four PDQFs at once, the sums of each search split over
+// the two halves of the wave: group s = (lane >> 3) & 3 of the lower half
+// accumulates the elements above the guess, the same group of the upper half
+// the elements below it, so each lane's sequential sum covers one side (half
+// the instructions per element).  The upper half works on negated elements,
+// y = -x, where x < guess and x >= mn read y > -guess and y <= -mn -- the
+// same two comparisons as the lower half with other bounds.  IEEE negation
+// is exact and round-to-nearest is symmetric, so the sum of negated elements
+// in the reference's order is the negated sum, except that an exactly-zero
+// sum is +0 both ways: the lower side's sum is recovered as +0 or as the
+// negation.  Counts and sums cross halves with one lane swap each.
+__device__ __forceinline__ void median_search8_split(const float* __restrict__ P, int lane, float& threshold,
+                                                     float& bit_threshold) {
+  const int l = lane & 7;
+  const bool lower_side = lane >= 32;
+  float x[32];
+#pragma unroll
+  for (int c = 0; c < 32; c++) x[c] = P[8 * c + l];
+  float mxl = -3.40282347e+38f, mnl = 3.40282347e+38f;  // f32::MIN, f32::MAX
+#pragma unroll
+  for (int c = 0; c < 32; c++) {
+    mnl = x86min(mnl, x[c]);
+    mxl = x86max(mxl, x[c]);
+  }
+  float mx = grp_max8(mxl, lane);
+  float mn = grp_min8(mnl, lane);
+#pragma unroll
+  for (int c = 0; c < 32; c++) x[c] = lower_side ? -x[c] : x[c];
+  float guess = (mn + mx) * 0.5f;
+  float last = guess;
+  uint32_t num_over = 0, num_under = 0;
+  bool done = false;
+#pragma unroll 1
+  for (int iter = 0; iter < 32; iter++) {
+    // this half's side: (guess, mx] above, or [mn, guess) below (negated)
+    const float lo_b = lower_side ? -guess : guess;
+    const float hi_b = lower_side ? -mn : mx;
+    float r = 0.0f;
+    uint32_t n = 0;
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+      const float v = x[c];
+      const bool m = (v > lo_b) && (v <= hi_b);
+      n += m ? 1u : 0u;
+      r = r + (m ? v : 0.0f);
+    }
+    const uint32_t cnt = grp_isum8(n);
+    float sum = grp_sum8(r);
+    if (lower_side) sum = sum == 0.0f ? 0.0f : -sum;  // back to the original sign (+0 for zero)
+    const uint32_t cnt_o = half_swap(cnt, lane);
+    const float sum_o = __uint_as_float(half_swap(__float_as_uint(sum), lane));
+    const uint32_t gtc = lower_side ? cnt_o : cnt, ltc = lower_side ? cnt : cnt_o;
+    const float sg = lower_side ? sum_o : sum, sl = lower_side ? sum : sum_o;
+    if (!done) {
+      last = guess;
+      if (gtc + num_over > 128) {
+        num_under += ltc;
+        mn = guess;
+        guess = sg / (float)gtc;
+      } else if (ltc + num_under > 128) {
+        num_over += gtc;
+        mx = guess;
+        guess = sl / (float)ltc;
+      } else {
+        done = true;
+      }
+    }
+    if (wave_all(done)) break;
+  }
+  threshold = guess;
+  bit_threshold = last;
+}
+
 // Byte b (0..31) of the hash of P at bit threshold t: bit k is
 // P[8(31-b) + k] > t (element i at byte 31 - i/8, bit i%8; x86.rs:497-503,
 // threshold.rs:45-115).

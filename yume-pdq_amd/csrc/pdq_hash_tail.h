@@ -58,52 +58,56 @@ __device__ __forceinline__ void stage2_to_pdqf(const float* __restrict__ d2, Wav
   wave_lds_sync();
 }
 
-// From the NORMAL PDQF in W.p: the eight dihedral PDQFs, their qualities,
-// thresholds and hashes.  thr_normal < 0 (null) means "use NORMAL's own
-// quantisation threshold" for ROTATED_90_FLOPPED; the visit entry point passes
-// the caller's threshold instead (src/lib.rs:403-404).
-__device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mode, const HashOutputs& out, uint32_t img,
-                              const float* thr_normal) {
-  // The eight dihedral PDQFs into W.t (d-major, 256 each):
-  //   0 NORMAL P          1 FLOPPED  sc.P       2 ROTATED_180 sr.sc.P   3 FLIPPED sr.P
-  //   4 ROT90_FLOPPED P^T 5 ROT270   sr.P^T     6 FLOPPED_ROT270 sr.sc.P^T  7 ROT90 sc.P^T
-  // sr negates even rows (pdqf_negate_alt_rows::<true>, x86.rs:285-326); sc
-  // negates even columns (pdqf_negate_alt_cols::<false>) -- all 16 in exact
-  // mode, columns 0..7 only in the AVX2-compatible mode (x86.rs:330-376 loads
-  // 8 floats per row).  Negation is exact, so the sequence of in-place flips
-  // in visit_dihedrals reduces to these closed forms.
-  {
-    const int ncols = mode == kDihedralAvx2 ? 8 : 16;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int idx = lane + 64 * e;  // element (i, j) of the output orientation
-      const int i = idx >> 4, j = idx & 15;
-      const float a = W.p[idx];
-      const float at = W.p[j * 16 + i];
-      const bool nr = (i & 1) == 0;
-      const bool nc = (j & 1) == 0 && j < ncols;
-      W.t[0 * 256 + idx] = a;
-      W.t[1 * 256 + idx] = nc ? a * -1.0f : a;
-      W.t[2 * 256 + idx] = (nr != nc) ? a * -1.0f : a;
-      W.t[3 * 256 + idx] = nr ? a * -1.0f : a;
-      W.t[4 * 256 + idx] = at;
-      W.t[5 * 256 + idx] = nr ? at * -1.0f : at;
-      W.t[6 * 256 + idx] = (nr != nc) ? at * -1.0f : at;
-      W.t[7 * 256 + idx] = nc ? at * -1.0f : at;
-    }
+// The eight dihedral PDQFs into W.t (d-major, 256 each), from the NORMAL
+// PDQF in W.p, element idx of orientations d0 .. d0 + nd - 1:
+//   0 NORMAL P          1 FLOPPED  sc.P       2 ROTATED_180 sr.sc.P   3 FLIPPED sr.P
+//   4 ROT90_FLOPPED P^T 5 ROT270   sr.P^T     6 FLOPPED_ROT270 sr.sc.P^T  7 ROT90 sc.P^T
+// sr negates even rows (pdqf_negate_alt_rows::<true>, x86.rs:285-326); sc
+// negates even columns (pdqf_negate_alt_cols::<false>) -- all 16 in exact
+// mode, columns 0..7 only in the AVX2-compatible mode (x86.rs:330-376 loads
+// 8 floats per row).  Negation is exact, so the sequence of in-place flips
+// in visit_dihedrals reduces to these closed forms.
+__device__ __forceinline__ void dihedral_pdqf_element(WaveLds& W, int idx, int d0, int nd, uint32_t mode) {
+  const int ncols = mode == kDihedralAvx2 ? 8 : 16;
+  const int i = idx >> 4, j = idx & 15;
+  const float a = W.p[idx];
+  const float at = W.p[j * 16 + i];
+  const bool nr = (i & 1) == 0;
+  const bool nc = (j & 1) == 0 && j < ncols;
+  for (int d = d0; d < d0 + nd; d++) {
+    const float v = d < 4 ? a : at;
+    const int kind = d & 3;  // 0: as is, 1: sc (FLOPPED / ROT90), 2: sr.sc, 3: sr
+    // the transposed four negate (none, sr, sr.sc, sc) for d = 4, 5, 6, 7
+    const int k = d < 4 ? kind : (kind == 1 ? 3 : kind == 3 ? 1 : kind);
+    const bool neg = k == 1 ? nc : k == 2 ? (nr != nc) : k == 3 ? nr : false;
+    W.t[d * 256 + idx] = neg ? v * -1.0f : v;
   }
-  wave_lds_sync();
+}
 
-  // Gradient sums (x86.rs:840-865), one sequential accumulation per
-  // dihedral, lane d owns dihedral d.
-  if (lane < 8) W.q[lane] = adjust_quality(sum_of_gradients16(&W.t[lane * 256]));
+// Quality of the eight orientations (gradient sums, x86.rs:840-865, one
+// sequential accumulation per orientation, lane d owns orientation d) into
+// W.q and, when write_out, the output.
+__device__ __forceinline__ void tail_quality(WaveLds& W, int lane, const HashOutputs& out, uint32_t img,
+                                             bool write_out) {
+  if (lane < 8) {
+    const float q = adjust_quality(sum_of_gradients16(&W.t[lane * 256]));
+    W.q[lane] = q;
+    if (write_out) out.quality[(size_t)img * 8 + lane] = q;
+  }
+}
 
-  // Median search (x86.rs:440-621) for dihedrals 0..3: group s = lane >> 3
-  // (lanes 32..63 repeat groups 0..3).
+// Thresholds and hashes of the eight orientations in W.t, and the outputs
+// other than quality.  thr_normal null means "use NORMAL's own quantisation
+// threshold" for ROTATED_90_FLOPPED; the visit entry point passes the
+// caller's threshold instead (src/lib.rs:403-404).
+__device__ __forceinline__ void tail_thresholds_bits(WaveLds& W, int lane, const HashOutputs& out, uint32_t img,
+                                                     const float* thr_normal) {
+  // Median search (x86.rs:440-621) for dihedrals 0..3: group s =
+  // (lane >> 3) & 3, the two halves of the wave summing the two sides.
   {
     const int s = (lane >> 3) & 3;
     float thr, bthr;
-    median_search8(&W.t[s * 256], lane, thr, bthr);
+    median_search8_split(&W.t[s * 256], lane, thr, bthr);
     if (lane < 32 && (lane & 7) == 0) {
       W.th[s] = thr;
       W.bt[s] = bthr;
@@ -129,10 +133,7 @@ __device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mod
     for (int q = 0; q < 4; q++) word |= hash_byte(P, 4 * w + q, bt) << (8 * q);
     reinterpret_cast<uint32_t*>(out.hashes + (size_t)img * 256 + d * 32)[w] = word;
   }
-  if (lane < 8) {
-    out.quality[(size_t)img * 8 + lane] = W.q[lane];
-    out.threshold[(size_t)img * 8 + lane] = W.th[lane];
-  }
+  if (lane < 8) out.threshold[(size_t)img * 8 + lane] = W.th[lane];
   if (out.pdqf) {
 #pragma unroll
     for (int e = 0; e < 4; e++) out.pdqf[(size_t)img * 256 + lane + 64 * e] = W.p[lane + 64 * e];
@@ -143,6 +144,17 @@ __device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mod
     for (int e = 0; e < 32; e++) dst[lane + 64 * e] = W.t[lane + 64 * e];
   }
   wave_lds_sync();
+}
+
+// The whole tail on one wave: from the NORMAL PDQF in W.p, the eight
+// dihedral PDQFs, their qualities, thresholds and hashes.
+__device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mode, const HashOutputs& out, uint32_t img,
+                              const float* thr_normal) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) dihedral_pdqf_element(W, lane + 64 * e, 0, 8, mode);
+  wave_lds_sync();
+  tail_quality(W, lane, out, img, true);
+  tail_thresholds_bits(W, lane, out, img, thr_normal);
 }
 
 }  // namespace ypdq
