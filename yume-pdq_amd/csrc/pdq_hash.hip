@@ -143,7 +143,7 @@ __device__ __forceinline__ void write_stage1(const float (&acc)[16][2][2], WaveL
       a1 = __builtin_isnan(a1) ? 0.0f : a1;
       const float v = a0 + a1;
       const int j = 2 * lane + jj;
-      W.t[k * 128 + j] = (j == 127) ? 0.0f : v;
+      W.t[k * kTS + j] = (j == 127) ? 0.0f : v;
     }
   }
 }
@@ -201,7 +201,7 @@ __device__ __noinline__ void stream_image_literal(const T* __restrict__ plane, i
         a1 = __builtin_isnan(a1) ? 0.0f : a1;
         const float v = a0 + a1;
         const int j = 2 * lane + jj;
-        W.t[(k0 + k) * 128 + j] = (j == 127) ? 0.0f : v;
+        W.t[(k0 + k) * kTS + j] = (j == 127) ? 0.0f : v;
       }
     }
   }
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) p
   // the next row's first coefficient, or the zero pad after row 15)
   for (int e = threadIdx.x; e < 16 * 128; e += blockDim.x) {
     const int j = e >> 7, m = e & 127;
-    L.d2[e] = kDct[j * 127 + m];
+    L.d2[j * kTS + m] = kDct[j * 127 + m];
   }
   __syncthreads();
   const uint32_t img = blockIdx.x * kWavesPerBlock + wv;

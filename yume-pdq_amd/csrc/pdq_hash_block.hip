@@ -37,7 +37,7 @@ namespace {
 
 struct ImageTail {
   WaveLds w;
-  float d2[16 * 128];
+  float d2[16 * kTS];  // DCT rows, stride kTS
 };
 
 struct ImageLds {
@@ -45,6 +45,7 @@ struct ImageLds {
     float c[128 * 128];  // compressed plane, row-major, stride 128
     ImageTail tail;      // after stage 1
   } u;
+  float dct_t[127 * 16];  // exact stage 1: D transposed, dct_t[k2 * 16 + k] = D[k][k2]
 };
 
 // Rows in flight per wave in the band loop: D compressed-row steps of four
@@ -146,27 +147,78 @@ __device__ __forceinline__ bool edges_nonfinite(const float* __restrict__ plane,
 // ---------------------------------------------------------------------------
 // B. stage 1, exact: thread (column j, frequencies k0 .. k0+NK-1).
 // ---------------------------------------------------------------------------
-template <int P, int NK>
-__device__ __forceinline__ void stage1_exact_step(float cv, const float* __restrict__ dk, float (&acc)[NK][2]) {
-#pragma unroll
-  for (int k = 0; k < NK; k++) acc[k][P] = __builtin_fmaf(cv, dk[k], acc[k][P]);
-}
+// The plane and the transposed DCT both sit in LDS, so every operand load is
+// an LDS load (those complete in order) and the loads of the next chunk of
+// rows are issued before the FMAs of the current one.
+template <int NK>
+struct DctSlice {
+  float v[NK];
+};
 
 template <int NK>
-__device__ __forceinline__ void stage1_exact(const float* __restrict__ c, int j, int k0, float (&acc)[NK][2]) {
+__device__ __forceinline__ void load_slice(const float* __restrict__ p, DctSlice<NK>& s) {
+  if constexpr (NK % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < NK / 4; q++) {
+      const f32x4 v = reinterpret_cast<const f32x4*>(p)[q];
+      s.v[4 * q] = v.x; s.v[4 * q + 1] = v.y; s.v[4 * q + 2] = v.z; s.v[4 * q + 3] = v.w;
+    }
+  } else {
+    static_assert(NK == 2, "two, four or eight frequencies per thread");
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    s.v[0] = v.x; s.v[1] = v.y;
+  }
+}
+
+template <int NK, int R>
+__device__ __forceinline__ void stage1_load_chunk(const float* __restrict__ c, const float* __restrict__ dt, int i0,
+                                                  float (&cv)[R], DctSlice<NK> (&dv)[R], int n = R) {
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (r < n) {
+      cv[r] = c[(i0 + r) * 128];
+      load_slice<NK>(dt + (i0 + r) * 16, dv[r]);
+    }
+}
+
+// Rows i0 .. i0 + n - 1 with i0 a multiple of 4, so row i0 + r takes
+// accumulator alt(r).
+template <int NK, int R>
+__device__ __forceinline__ void stage1_fma_chunk(const float (&cv)[R], const DctSlice<NK> (&dv)[R],
+                                                 float (&acc)[NK][2], int n = R) {
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (r < n) {
+      const int p = alt(r);
+#pragma unroll
+      for (int k = 0; k < NK; k++) acc[k][p] = __builtin_fmaf(cv[r], dv[r].v[k], acc[k][p]);
+    }
+}
+
+// acc[k][p] = the reference's two alternating chains over k2 = 0..126
+// (x86.rs:161-225) for output (k0 + k, j); c is column j of the plane, dt
+// the transposed DCT at frequency k0.  Chunks of eight rows, the loads of
+// chunk b + 1 in flight while chunk b is accumulated.
+template <int NK>
+__device__ __forceinline__ void stage1_exact(const float* __restrict__ c, const float* __restrict__ dt,
+                                             float (&acc)[NK][2]) {
+  constexpr int R = 8;
 #pragma unroll
   for (int k = 0; k < NK; k++) acc[k][0] = acc[k][1] = 0.0f;
-  const float* dk = kDctT.v + k0;
+  float cv0[R], cv1[R];
+  DctSlice<NK> dv0[R], dv1[R];
+  stage1_load_chunk<NK, R>(c, dt, 0, cv0, dv0);
 #pragma unroll 1
-  for (int i = 0; i < 124; i += 4) {
-    stage1_exact_step<1>(c[i * 128 + j], dk + i * 16, acc);
-    stage1_exact_step<0>(c[(i + 1) * 128 + j], dk + (i + 1) * 16, acc);
-    stage1_exact_step<0>(c[(i + 2) * 128 + j], dk + (i + 2) * 16, acc);
-    stage1_exact_step<1>(c[(i + 3) * 128 + j], dk + (i + 3) * 16, acc);
+  for (int i0 = 0; i0 < 112; i0 += 2 * R) {
+    stage1_load_chunk<NK, R>(c, dt, i0 + R, cv1, dv1);
+    stage1_fma_chunk<NK, R>(cv0, dv0, acc);
+    stage1_load_chunk<NK, R>(c, dt, i0 + 2 * R, cv0, dv0);
+    stage1_fma_chunk<NK, R>(cv1, dv1, acc);
   }
-  stage1_exact_step<1>(c[124 * 128 + j], dk + 124 * 16, acc);
-  stage1_exact_step<0>(c[125 * 128 + j], dk + 125 * 16, acc);
-  stage1_exact_step<0>(c[126 * 128 + j], dk + 126 * 16, acc);
+  // rows 112 .. 119 are in cv0; rows 120 .. 126 remain
+  stage1_load_chunk<NK, R>(c, dt, 120, cv1, dv1, 7);
+  stage1_fma_chunk<NK, R>(cv0, dv0, acc);
+  stage1_fma_chunk<NK, R>(cv1, dv1, acc, 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -293,7 +345,7 @@ __device__ __forceinline__ void stage2_mfma(WaveLds& W, int lane, const typename
   f32x4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int s = 0; s < 4; s++) {
-    const f32x4* tp = reinterpret_cast<const f32x4*>(&W.t[m * 128 + 32 * s + 8 * g]);
+    const f32x4* tp = reinterpret_cast<const f32x4*>(&W.t[m * kTS + 32 * s + 8 * g]);
     const f32x4 ta = tp[0], tb = tp[1];
     const float x[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
     typename Precision<kPrec>::Op a;
@@ -325,6 +377,8 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const T* plane = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(planes) + (size_t)img * plane_stride);
   float* c = L.u.c;
+  if (kPrec == kDctExact || std::is_same<T, float>::value)
+    for (int e = t; e < 127 * 16; e += 64 * kWaves) L.dct_t[e] = kDctT.v[e];
 
   // A. compression
   const int i0 = wv * kBand, i1 = min(i0 + kBand, 127);
@@ -357,14 +411,14 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
     const int j = t & 127;
     const int k0 = kNK * __builtin_amdgcn_readfirstlane(t >> 7);
     float acc[kNK][2];
-    stage1_exact<kNK>(c, j, k0, acc);
+    stage1_exact<kNK>(c + j, L.dct_t + k0, acc);
     __syncthreads();  // every read of the plane is done: T and d2 overwrite it
 #pragma unroll
     for (int k = 0; k < kNK; k++) {
       float a0 = acc[k][0], a1 = acc[k][1];
       a0 = __builtin_isnan(a0) ? 0.0f : a0;
       a1 = __builtin_isnan(a1) ? 0.0f : a1;
-      W.t[(k0 + k) * 128 + j] = (j == 127) ? 0.0f : a0 + a1;
+      W.t[(k0 + k) * kTS + j] = (j == 127) ? 0.0f : a0 + a1;
     }
   } else {
     // B'. stage 1 on matrix cores: wave w computes column tiles w, w + kWaves
@@ -379,12 +433,12 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
 #pragma unroll
     for (int q = 0; q < kTiles; q++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) W.t[(4 * g + r) * 128 + 16 * (wv + kWaves * q) + nn] = acc[q][r];
+      for (int r = 0; r < 4; r++) W.t[(4 * g + r) * kTS + 16 * (wv + kWaves * q) + nn] = acc[q][r];
   }
   if (kPrec == kDctExact || literal) {
     for (int e = t; e < 16 * 128; e += 64 * kWaves) {
       const int jj = e >> 7, m = e & 127;
-      d2[e] = kDct[jj * 127 + m];
+      d2[jj * kTS + m] = kDct[jj * 127 + m];
     }
   }
   __syncthreads();

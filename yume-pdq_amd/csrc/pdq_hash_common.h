@@ -286,8 +286,13 @@ __device__ __forceinline__ bool nonfinite(float x) { return !__builtin_isfinite(
 // ---------------------------------------------------------------------------
 // Per-wave LDS layout.
 // ---------------------------------------------------------------------------
+// Row stride of the stage-1 rows and of the DCT rows in LDS: 128 columns
+// plus four pad floats, so the sixteen rows one ds_read_b128 of stage 2
+// touches start in different banks.
+constexpr int kTS = 132;
+
 struct WaveLds {
-  float t[16 * 128];  // stage-1 rows; later the 8 dihedral PDQFs (8 x 256)
+  float t[16 * kTS];  // stage-1 rows (stride kTS); later the 8 dihedral PDQFs (8 x 256)
   float p[256];       // PDQF of the NORMAL orientation
   float q[8];         // per-dihedral quality
   float th[8];        // per-dihedral threshold
@@ -297,7 +302,7 @@ struct WaveLds {
 constexpr int kWavesPerBlock = 4;
 
 struct BlockLds {
-  float d2[16 * 128];  // DCT rows padded to 128 (slot 127 = the reference's over-read)
+  float d2[16 * kTS];  // DCT rows padded to 128 (slot 127 = the reference's over-read), stride kTS
   WaveLds w[kWavesPerBlock];
 };
 

@@ -15,8 +15,8 @@ namespace ypdq {
 __device__ __forceinline__ void write_tmp(const WaveLds& W, int lane, const HashOutputs& out, uint32_t img) {
   if (out.tmp) {
     float* dst = out.tmp + (size_t)img * 128;
-    dst[2 * lane] = W.t[15 * 128 + 2 * lane];
-    dst[2 * lane + 1] = W.t[15 * 128 + 2 * lane + 1];
+    dst[2 * lane] = W.t[15 * kTS + 2 * lane];
+    dst[2 * lane + 1] = W.t[15 * kTS + 2 * lane + 1];
   }
 }
 
@@ -34,8 +34,8 @@ __device__ __forceinline__ void stage2_column(const float* __restrict__ d2, Wave
     for (int l = 0; l < 8; l++) h[p][l] = 0.0f;
 #pragma unroll
   for (int c = 0; c < 16; c++) {
-    const f32x4* tp = reinterpret_cast<const f32x4*>(&W.t[k * 128 + 8 * c]);
-    const f32x4* dp = reinterpret_cast<const f32x4*>(&d2[j * 128 + 8 * c]);
+    const f32x4* tp = reinterpret_cast<const f32x4*>(&W.t[k * kTS + 8 * c]);
+    const f32x4* dp = reinterpret_cast<const f32x4*>(&d2[j * kTS + 8 * c]);
     const f32x4 ta = tp[0], tb = tp[1], da = dp[0], db = dp[1];
     const float tv[8] = {ta.x, ta.y, ta.z, ta.w, tb.x, tb.y, tb.z, tb.w};
     const float dv[8] = {da.x, da.y, da.z, da.w, db.x, db.y, db.z, db.w};
