@@ -461,9 +461,21 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
     dihedral_pdqf_element(W, idx, d0, kPer, mode);
   }
   __syncthreads();
-  // the quality on wave 1 while wave 0 searches the thresholds
-  if (wv == 1) tail_quality(W, lane, out, img, true);
-  if (wv == 0) tail_thresholds_bits(W, lane, out, img, nullptr);
+  if constexpr (kWaves == 8) {
+    // waves 0..3 search the four thresholds and write the hashes, wave 4
+    // the qualities, waves 5..7 the optional PDQF outputs
+    if (wv < 4) {
+      tail_rows_thresholds_bits(W, lane, wv, out, img, nullptr);
+    } else if (wv == 4) {
+      tail_quality(W, lane, out, img, true);
+    } else {
+      tail_copy_pdqfs(W, t - 5 * 64, 3 * 64, out, img);
+    }
+  } else {
+    // the quality on wave 1 while wave 0 searches the thresholds
+    if (wv == 1) tail_quality(W, lane, out, img, true);
+    if (wv == 0) tail_thresholds_bits(W, lane, out, img, nullptr);
+  }
 }
 
 template <typename T, int kPrec, int kWaves>

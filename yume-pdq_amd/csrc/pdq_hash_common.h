@@ -307,36 +307,53 @@ struct BlockLds {
 };
 
 // 8-lane group reductions with the AVX2 tree shapes.  Lanes of a group are
-// g*8 .. g*8+7; the value of the tree is produced in every lane by xor
-// shuffles, which for addition gives identical results in all lanes (IEEE
-// addition commutes) and for max/min is read from the group's lane 0.
+// g*8 .. g*8+7.  The operand moves are DPP lane permutations within rows of
+// 16 lanes: quad_perm [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1] (lane ^ 2),
+// row_half_mirror (lane 7 - i of the group) and row_ror:12 (lane i takes
+// lane i - 12 mod 16 = i + 4 for i < 12).  For
+// addition the tree's value appears in every lane (IEEE addition commutes,
+// and after two steps lanes 4..7 all hold the same upper half); max and min
+// are read from the group's lane 0, which takes the operands of the xor tree.
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(x), kCtrl, 0xf, 0xf, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __uint_as_float(dpp_u<kCtrl>(__float_as_uint(x)));
+}
+constexpr int kDppXor1 = 0xb1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4e;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
+constexpr int kDppPlus4 = 0x12c;      // row_ror:12
+
 __device__ __forceinline__ float grp_sum8(float a) {
-  float s1 = a + shfl_xor(a, 1);
-  float s2 = s1 + shfl_xor(s1, 2);
-  return s2 + shfl_xor(s2, 4);
+  const float s1 = a + dpp_f<kDppXor1>(a);
+  const float s2 = s1 + dpp_f<kDppXor2>(s1);
+  return s2 + dpp_f<kDppHalfMirror>(s2);
 }
 
 __device__ __forceinline__ float x86max(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float x86min(float a, float b) { return a < b ? a : b; }
 
 __device__ __forceinline__ float grp_max8(float a, int lane) {
-  float p = x86max(a, shfl_xor(a, 1));   // lane 2m: max(a_2m, a_2m+1)
-  float q = x86max(p, shfl_xor(p, 2));   // lanes 0,4: max(p0,p2), max(p4,p6)
-  float r = x86max(q, shfl_xor(q, 4));   // lane 0: max(q0, q4)
+  float p = x86max(a, dpp_f<kDppXor1>(a));  // lane 2m: max(a_2m, a_2m+1)
+  float q = x86max(p, dpp_f<kDppXor2>(p));  // lanes 0,4: max(p0,p2), max(p4,p6)
+  float r = x86max(q, dpp_f<kDppPlus4>(q));  // lane 0: max(q0, q4)
   return shfl_idx(r, lane & ~7);
 }
 
 __device__ __forceinline__ float grp_min8(float a, int lane) {
-  float p = x86min(a, shfl_xor(a, 1));
-  float q = x86min(p, shfl_xor(p, 2));
-  float r = x86min(q, shfl_xor(q, 4));
+  float p = x86min(a, dpp_f<kDppXor1>(a));
+  float q = x86min(p, dpp_f<kDppXor2>(p));
+  float r = x86min(q, dpp_f<kDppPlus4>(q));
   return shfl_idx(r, lane & ~7);
 }
 
 __device__ __forceinline__ uint32_t grp_isum8(uint32_t a) {
-  a += shfl_xor_u(a, 1);
-  a += shfl_xor_u(a, 2);
-  return a + shfl_xor_u(a, 4);
+  a += dpp_u<kDppXor1>(a);
+  a += dpp_u<kDppXor2>(a);
+  return a + dpp_u<kDppHalfMirror>(a);
 }
 
 
@@ -518,6 +535,111 @@ __device__ __forceinline__ void median_search8_split(const float* __restrict__ P
       }
     }
     if (wave_all(done)) break;
+  }
+  threshold = guess;
+  bit_threshold = last;
+}
+
+// The value of the same lane in the partner row of 16 lanes, rows 0 <-> 1
+// and 2 <-> 3 (v_permlane16_swap: the odd rows of the first operand trade
+// places with the even rows of the second).
+__device__ __forceinline__ uint32_t row_swap(uint32_t x, int lane) {
+  const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+  return (lane & 16) ? r[0] : r[1];
+}
+__device__ __forceinline__ float row_swapf(float x, int lane) {
+  return __uint_as_float(row_swap(__float_as_uint(x), lane));
+}
+__device__ __forceinline__ float half_swapf(float x, int lane) {
+  return __uint_as_float(half_swap(__float_as_uint(x), lane));
+}
+
+// median_search8 of one PDQF on a whole wave, for latency: each of the
+// sixteen sequential sums (8 AVX lanes x the two sides of the guess, as in
+// median_search8_split) is cut into four runs of 8 elements on four rows of
+// the wave.  Every lane selects its run's elements and counts them at once;
+// then the runs are summed in order, row 0, 1, 3, 2, each row starting from
+// the previous row's partial sum (v_permlane16/32_swap move it), so every
+// sum adds the same values in the same order as the reference's AVX lane.
+// Lane layout: l = lane & 7 the AVX lane, bit 3 the side (set: below the
+// guess, on negated elements), row = lane >> 4 holds run row ^ (row >> 1).
+// The counts are integers and are summed in any order.  The guess and the
+// bounds are wave-uniform, so the search's branches are scalar.
+__device__ __forceinline__ void median_search_rows(const float* __restrict__ P, int lane, float& threshold,
+                                                   float& bit_threshold) {
+  const int l = lane & 7;
+  const bool lower_side = (lane & 8) != 0;
+  const int row = lane >> 4;
+  const int run = row ^ (row >> 1);
+  float mx, mn;
+  {
+    // the reference's running extremes over the 32 elements of AVX lane l
+    float mxl = -3.40282347e+38f, mnl = 3.40282347e+38f;  // f32::MIN, f32::MAX
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+      const float v = P[8 * c + l];
+      mnl = x86min(mnl, v);
+      mxl = x86max(mxl, v);
+    }
+    mx = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(grp_max8(mxl, lane))));
+    mn = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(grp_min8(mnl, lane))));
+  }
+  float x[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const float v = P[8 * (8 * run + e) + l];
+    x[e] = lower_side ? -v : v;
+  }
+  float guess = (mn + mx) * 0.5f;
+  float last = guess;
+  uint32_t num_over = 0, num_under = 0;
+#pragma unroll 1
+  for (int iter = 0; iter < 32; iter++) {
+    // this lane's side: (guess, mx] above, or [mn, guess) below (negated)
+    const float lo_b = lower_side ? -guess : guess;
+    const float hi_b = lower_side ? -mn : mx;
+    float s[8];
+    uint32_t n = 0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const bool m = (x[e] > lo_b) && (x[e] <= hi_b);
+      n += m ? 1u : 0u;
+      s[e] = m ? x[e] : 0.0f;
+    }
+    // the sequential sum, run after run
+    float acc = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; e++) acc = acc + s[e];
+    acc = row_swapf(acc, lane);  // row 1 continues row 0
+#pragma unroll
+    for (int e = 0; e < 8; e++) acc = acc + s[e];
+    acc = half_swapf(acc, lane);  // row 3 continues row 1
+#pragma unroll
+    for (int e = 0; e < 8; e++) acc = acc + s[e];
+    acc = row_swapf(acc, lane);  // row 2 continues row 3
+#pragma unroll
+    for (int e = 0; e < 8; e++) acc = acc + s[e];
+    float sum = grp_sum8(acc);  // the AVX tree; rows 2 hold the whole sums
+    if (lower_side) sum = sum == 0.0f ? 0.0f : -sum;  // back to the original sign (+0 for zero)
+    n = grp_isum8(n);
+    n += row_swap(n, lane);
+    n += half_swap(n, lane);
+    const uint32_t gtc = __builtin_amdgcn_readlane(n, 0);
+    const uint32_t ltc = __builtin_amdgcn_readlane(n, 8);
+    const float sg = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(sum), 32));
+    const float sl = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(sum), 40));
+    last = guess;
+    if (gtc + num_over > 128) {
+      num_under += ltc;
+      mn = guess;
+      guess = sg / (float)gtc;
+    } else if (ltc + num_under > 128) {
+      num_over += gtc;
+      mx = guess;
+      guess = sl / (float)ltc;
+    } else {
+      break;
+    }
   }
   threshold = guess;
   bit_threshold = last;

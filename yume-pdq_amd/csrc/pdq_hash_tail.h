@@ -146,6 +146,36 @@ __device__ __forceinline__ void tail_thresholds_bits(WaveLds& W, int lane, const
   wave_lds_sync();
 }
 
+// The latency form of the thresholds and bits, on four waves: wave s
+// (0..3) searches the threshold of orientation s on all its lanes
+// (median_search_rows) and writes hash s at s's bit threshold and hash s + 4
+// at s's threshold -- the caller's for s == 0 when thr_normal is given
+// (src/lib.rs:403-416, as tail_thresholds_bits) -- one byte per lane.
+__device__ __forceinline__ void tail_rows_thresholds_bits(WaveLds& W, int lane, int s, const HashOutputs& out,
+                                                          uint32_t img, const float* thr_normal) {
+  float thr, bthr;
+  median_search_rows(&W.t[s * 256], lane, thr, bthr);
+  const float t4 = (s == 0 && thr_normal) ? *thr_normal : thr;
+  const int d = lane < 32 ? s : s + 4, b = lane & 31;
+  out.hashes[(size_t)img * 256 + d * 32 + b] = (uint8_t)hash_byte(&W.t[d * 256], b, lane < 32 ? bthr : t4);
+  if (lane == 0) {
+    out.threshold[(size_t)img * 8 + s] = thr;
+    out.threshold[(size_t)img * 8 + s + 4] = t4;
+  }
+}
+
+// The optional PDQF outputs (NORMAL's, and all eight orientations'), thread
+// t0 of n.
+__device__ __forceinline__ void tail_copy_pdqfs(const WaveLds& W, int t0, int n, const HashOutputs& out,
+                                                uint32_t img) {
+  if (out.pdqf)
+    for (int e = t0; e < 256; e += n) out.pdqf[(size_t)img * 256 + e] = W.p[e];
+  if (out.dihedral_pdqf) {
+    float* dst = out.dihedral_pdqf + (size_t)img * 8 * 256;
+    for (int e = t0; e < 8 * 256; e += n) dst[e] = W.t[e];
+  }
+}
+
 // The whole tail on one wave: from the NORMAL PDQF in W.p, the eight
 // dihedral PDQFs, their qualities, thresholds and hashes.
 __device__ __forceinline__ void dihedral_tail(WaveLds& W, int lane, uint32_t mode, const HashOutputs& out, uint32_t img,
