@@ -45,7 +45,7 @@ namespace {
 
 using ypdq::HashOutputs;
 
-// Device-side result block of one drop-in call, copied back in one transfer.
+// Result block of one drop-in call.
 struct DropInResult {
   uint8_t hashes[8 * 32];
   float quality[8];
@@ -56,14 +56,33 @@ struct DropInResult {
   float dihedral_pdqf[8 * 256];
 };
 
-// Per-thread device state for the drop-in entry points (the reference clones
-// its kernel per call, src/lib.rs:266, so calls are reentrant).
+// Per-thread state for the drop-in entry points (the reference clones its
+// kernel per call, src/lib.rs:266, so calls are reentrant).  A hash call
+// copies the image into page-locked h_img with memcpy; a copy kernel on the
+// call's stream pulls it into d_img (a DMA-engine copy followed by a kernel
+// costs a cross-engine dependency of a few tens of microseconds); the hash
+// kernel writes its results straight into page-locked h_res over the bus: no
+// copy back, one synchronisation per call.  The visit call's 1 KiB PDQF is
+// read by its kernel from h_pdqf directly.
 struct DropInState {
   hipStream_t stream = nullptr;
-  float* d_in = nullptr;  // 512 x 512 f32, or 256 PDQF + 1 threshold
-  DropInResult* d_res = nullptr;
-  DropInResult* h_res = nullptr;  // pinned
+  float* h_img = nullptr;         // 512 x 512 f32, page-locked, coherent
+  float* d_img = nullptr;         // its device copy
+  float* h_pdqf = nullptr;        // 256 PDQF + 1 threshold, page-locked, coherent
+  DropInResult* h_res = nullptr;  // the kernels' outputs, page-locked, coherent
+  float* dv_img = nullptr;        // device addresses of the page-locked three
+  float* dv_pdqf = nullptr;
+  DropInResult* dv_res = nullptr;
 };
+
+typedef float pull_f4 __attribute__((ext_vector_type(4)));
+
+// dst[i] = src[i] for n 16-byte elements, src in page-locked host memory.
+__global__ void __launch_bounds__(256) pull_plane(pull_f4* __restrict__ dst, const pull_f4* __restrict__ src,
+                                                  uint32_t n) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    dst[i] = __builtin_nontemporal_load(src + i);
+}
 
 std::mutex g_pool_mu;
 std::vector<DropInState*> g_pool;
@@ -77,10 +96,22 @@ DropInState* state_new() {
   DropInState* s = new DropInState();
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) die("hipStreamCreate", e);
-  if ((e = hipMalloc(&s->d_in, 512 * 512 * sizeof(float))) != hipSuccess) die("hipMalloc", e);
-  if ((e = hipMalloc(&s->d_res, sizeof(DropInResult))) != hipSuccess) die("hipMalloc", e);
-  if ((e = hipHostMalloc(&s->h_res, sizeof(DropInResult), hipHostMallocDefault)) != hipSuccess)
+  if ((e = hipMalloc(&s->d_img, 512 * 512 * sizeof(float))) != hipSuccess) die("hipMalloc", e);
+  // coherent (fine-grained): the GPU does not cache these lines, so a kernel
+  // reads the bytes just written by memcpy and its stores are in host memory
+  // when the stream synchronisation returns
+  if ((e = hipHostMalloc(&s->h_img, 512 * 512 * sizeof(float), hipHostMallocCoherent)) != hipSuccess)
     die("hipHostMalloc", e);
+  if ((e = hipHostMalloc(&s->h_pdqf, 257 * sizeof(float), hipHostMallocCoherent)) != hipSuccess)
+    die("hipHostMalloc", e);
+  if ((e = hipHostMalloc(&s->h_res, sizeof(DropInResult), hipHostMallocCoherent)) != hipSuccess)
+    die("hipHostMalloc", e);
+  if ((e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->dv_img), s->h_img, 0)) != hipSuccess)
+    die("hipHostGetDevicePointer", e);
+  if ((e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->dv_pdqf), s->h_pdqf, 0)) != hipSuccess)
+    die("hipHostGetDevicePointer", e);
+  if ((e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->dv_res), s->h_res, 0)) != hipSuccess)
+    die("hipHostGetDevicePointer", e);
   return s;
 }
 
@@ -142,17 +173,16 @@ float yume_pdq_hash_smart_kernel(const float* input, float* threshold, uint8_t* 
                                  float* pdqf) {
   DropInState* s = state();
   hipError_t e;
-  if ((e = hipMemcpyAsync(s->d_in, input, 512 * 512 * sizeof(float), hipMemcpyHostToDevice, s->stream)) !=
-      hipSuccess)
-    die("hipMemcpyAsync", e);
-  HashOutputs o = outputs_of(s->d_res, true);
+  memcpy(s->h_img, input, 512 * 512 * sizeof(float));
+  constexpr uint32_t kVec = 512 * 512 / 4;
+  hipLaunchKernelGGL(pull_plane, dim3(kVec / 256), dim3(256), 0, s->stream, reinterpret_cast<pull_f4*>(s->d_img),
+                     reinterpret_cast<const pull_f4*>(s->dv_img), kVec);
+  if ((e = hipGetLastError()) != hipSuccess) die("copy kernel launch", e);
+  HashOutputs o = outputs_of(s->dv_res, true);
   o.dihedral_pdqf = nullptr;
-  if ((e = ypdq::launch_hash(s->d_in, ypdq::kInputF32, 512 * 512 * sizeof(float), 1, ypdq::kDihedralAvx2, o,
+  if ((e = ypdq::launch_hash(s->d_img, ypdq::kInputF32, 512 * 512 * sizeof(float), 1, ypdq::kDihedralAvx2, o,
                              s->stream)) != hipSuccess)
     die("hash kernel launch", e);
-  if ((e = hipMemcpyAsync(s->h_res, s->d_res, offsetof(DropInResult, dihedral_pdqf), hipMemcpyDeviceToHost,
-                          s->stream)) != hipSuccess)
-    die("hipMemcpyAsync", e);
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) die("hipStreamSynchronize", e);
   const DropInResult* r = s->h_res;
   memcpy(output, r->hashes, 32);
@@ -167,19 +197,12 @@ bool yume_pdq_visit_dihedrals_smart_kernel(void* ctx, float threshold, uint8_t* 
                                            DihedralCallback callback) {
   DropInState* s = state();
   hipError_t e;
-  float* h_in = reinterpret_cast<float*>(s->h_res);  // staging: 256 PDQF + threshold
-  memcpy(h_in, pdqf, 256 * sizeof(float));
-  h_in[256] = threshold;
-  if ((e = hipMemcpyAsync(s->d_in, h_in, 257 * sizeof(float), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
-    die("hipMemcpyAsync", e);
-  // the staging area is overwritten by the result copy below, after the upload
-  HashOutputs o = outputs_of(s->d_res, false);
+  memcpy(s->h_pdqf, pdqf, 256 * sizeof(float));
+  s->h_pdqf[256] = threshold;
+  HashOutputs o = outputs_of(s->dv_res, false);
   o.pdqf = nullptr;
-  if ((e = ypdq::launch_dihedrals(s->d_in, s->d_in + 256, 1, ypdq::kDihedralAvx2, o, s->stream)) != hipSuccess)
+  if ((e = ypdq::launch_dihedrals(s->dv_pdqf, s->dv_pdqf + 256, 1, ypdq::kDihedralAvx2, o, s->stream)) != hipSuccess)
     die("dihedral kernel launch", e);
-  if ((e = hipMemcpyAsync(s->h_res, s->d_res, sizeof(DropInResult), hipMemcpyDeviceToHost, s->stream)) !=
-      hipSuccess)
-    die("hipMemcpyAsync", e);
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) die("hipStreamSynchronize", e);
   const DropInResult* r = s->h_res;
   // orientations 1..7 are the reference's visiting order (src/lib.rs:389-418)

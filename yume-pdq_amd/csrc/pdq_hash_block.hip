@@ -436,9 +436,11 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
       for (int r = 0; r < 4; r++) W.t[(4 * g + r) * kTS + 16 * (wv + kWaves * q) + nn] = acc[q][r];
   }
   if (kPrec == kDctExact || literal) {
+    // d2[j][m] = D[j][m] from the transposed copy in LDS; slot 127 holds the
+    // reference's over-read D_flat[127 j + 127] = D[j + 1][0] (0 after row 15)
     for (int e = t; e < 16 * 128; e += 64 * kWaves) {
-      const int jj = e >> 7, m = e & 127;
-      d2[jj * kTS + m] = kDct[jj * 127 + m];
+      const int jj = e & 15, m = e >> 4;
+      d2[jj * kTS + m] = m < 127 ? L.dct_t[e] : (jj < 15 ? L.dct_t[jj + 1] : 0.0f);
     }
   }
   __syncthreads();
