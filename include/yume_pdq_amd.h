@@ -491,7 +491,10 @@ int yume_pdq_amd_server_stats(const YumePdqAmdServer *server, YumePdqAmdServerSt
  * binary.  Returns OK at a clean end of input, YUME_PDQ_AMD_EEOF after
  * writing every whole frame when the input ends inside a frame,
  * YUME_PDQ_AMD_EIO on a read/write error, EINVAL for a bad format.
- * *frames: frames written (may be NULL).
+ * *frames: frames written (may be NULL).  The page-locked ring, the device
+ * buffers and the streams of a finished run are kept for the next run of the
+ * same batch_frames, input_kind and diagnostic-or-not (at most two such sets;
+ * yume_pdq_amd_pipe_trim frees them).
  */
 typedef struct YumePdqAmdPipeConfig {
   int input_fd;
@@ -504,6 +507,12 @@ typedef struct YumePdqAmdPipeConfig {
   int stats;             /* progress lines on stderr about once a second */
 } YumePdqAmdPipeConfig;
 int yume_pdq_amd_pipe_run(const YumePdqAmdPipeConfig *cfg, uint64_t *frames);
+
+/* Frees the buffer sets finished pipe runs keep (a set for batches of 256
+ * luma8 frames holds 192 MiB of page-locked and 128 MiB of device memory);
+ * runs in progress keep theirs.  Returns the number of sets freed.  No
+ * counterpart in the reference, whose pipe allocates per process. */
+int yume_pdq_amd_pipe_trim(void);
 
 /* Host-only formatting of hashed frames in a pipe output format (no GPU
  * call).  hashes n x 8 x 32, quality and threshold n x 8 as

@@ -140,3 +140,47 @@ def test_command_line(gpu, frames, tmp_path):
     assert len(r.stdout.decode().splitlines()) == 2
     r = subprocess.run([PIPE_BIN, "-f", "hexx"], input=b"", capture_output=True, timeout=60)
     assert r.returncode == 255
+
+
+def test_cached_buffers_reused(gpu, oracle_out, frames, tmp_path):
+    """A run keeps its buffer set for the next run of the same shape: two runs
+    in a row, a run after pipe_trim freed the cache, and runs of other shapes
+    in between all write the same bytes."""
+    data = frames.tobytes()
+    _, first = run_pipe(gpu, tmp_path, data, "q+hex+lf", dihedrals=True, batch=4)
+    _, again = run_pipe(gpu, tmp_path, data, "q+hex+lf", dihedrals=True, batch=4)
+    assert again == first
+    _, diag = run_pipe(gpu, tmp_path, data, "diagnostic+lf", dihedrals=True, batch=4)
+    _, other = run_pipe(gpu, tmp_path, data, "q+hex+lf", dihedrals=True, batch=3)
+    assert other == first
+    assert gpu.pipe_trim() >= 1
+    assert gpu.pipe_trim() == 0
+    _, after = run_pipe(gpu, tmp_path, data, "q+hex+lf", dihedrals=True, batch=4)
+    assert after == first
+    _, diag2 = run_pipe(gpu, tmp_path, data, "diagnostic+lf", dihedrals=True, batch=4)
+    assert diag2 == diag
+
+
+def test_batch_size_throughput(gpu, tmp_path):
+    """Frames per second do not fall from batches of 64 to batches of 256:
+    2048 frames from a file in the page cache, hex records to /dev/null, the
+    best of three runs per batch size after one run that sets up the cached
+    buffers."""
+    import time
+    base = synthetic.images_u8(64, start=500)
+    data = np.tile(base, (32, 1, 1)).tobytes()
+    src = tmp_path / "frames.raw"
+    src.write_bytes(data)
+    rate = {}
+    for batch in (64, 256):
+        gpu.pipe(str(src), os.devnull, fmt="hex+lf", batch=batch)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            n = gpu.pipe(str(src), os.devnull, fmt="hex+lf", batch=batch)
+            best = min(best, time.perf_counter() - t0)
+            assert n == 2048
+        rate[batch] = 2048 / best
+        say(f"[pipe] batch {batch}: {rate[batch]:.0f} frames/s")
+    gpu.pipe_trim()
+    assert rate[256] >= 0.9 * rate[64], rate

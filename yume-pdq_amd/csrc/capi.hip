@@ -575,6 +575,8 @@ int yume_pdq_amd_pipe_run(const YumePdqAmdPipeConfig* cfg, uint64_t* frames) {
   return YUME_PDQ_AMD_EHIP;
 }
 
+int yume_pdq_amd_pipe_trim(void) { return (int)ypdq::trim_pipe_cache(); }
+
 int64_t yume_pdq_amd_format_frames(const char* output_format, uint32_t n_frames, int dihedrals, const uint8_t* hashes,
                                    const float* quality, const float* threshold, const float* dihedral_pdqf, char* out,
                                    uint64_t out_capacity) {

@@ -49,4 +49,8 @@ struct PipeOptions {
 // *error).  *frames_out: frames written.
 int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error);
 
+// Frees the buffer sets that finished runs keep for the next run of the same
+// shape; returns how many were freed.
+size_t trim_pipe_cache();
+
 }  // namespace ypdq

@@ -8,6 +8,12 @@
 // kernel (input conversion fused into its row loader) and reads the hashes,
 // qualities and thresholds back, and while the GPU works on batch i + 1 it
 // formats and writes batch i.
+//
+// The page-locked ring, the device buffers, the streams and the events of a
+// run are kept for the next run of the same shape (batch size, frame size,
+// diagnostic output): pinning a ring of batches of 256 frames costs tens of
+// milliseconds, more than hashing a few thousand frames.  At most two idle
+// sets stay cached; trim_pipe_cache frees them.
 #include <errno.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -277,32 +283,61 @@ hipError_t device_view(T* host, T** dev) {
   return e;
 }
 
-}  // namespace
+// Everything a run needs besides its threads: the ring of page-locked
+// batches, two sets of device inputs and outputs, the streams and events.
+constexpr int kSlots = 3;
 
-int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
-  if (frames_out) *frames_out = 0;
-  const uint64_t frame_bytes = input_plane_bytes(o.input_kind);
-  if (frame_bytes == 0 || o.input_kind == kInputF32 || o.batch == 0) return -1;
-  const uint32_t B = o.batch;
-  const bool diag = o.format.type == OutputFormat::kDiagnostic;
-  const int kSlots = 3;
+struct PipeRes {
+  int device = 0;
+  uint32_t batch = 0;
+  uint64_t frame_bytes = 0;
+  bool diag = false;
   Slot slots[kSlots];
   Outputs outs[2];
   void* d_in[2] = {nullptr, nullptr};
-  hipStream_t stream = nullptr, upload = nullptr;  // hashing and readback; frame uploads
-  hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&upload, hipStreamNonBlocking);
   HostStores stores[2];
-  memset(stores, 0, sizeof stores);
-  auto fail_hip = [&](hipError_t err) {
-    if (error) *error = std::string("HIP: ") + hipGetErrorString(err);
-    return -2;
-  };
-  int rc = 0;
-  for (int s = 0; s < kSlots && e == hipSuccess; s++) e = hipHostMalloc((void**)&slots[s].frames, B * frame_bytes, 0);
+  hipStream_t stream = nullptr;  // hashing and readback
+  hipStream_t upload = nullptr;  // frame uploads
+};
+
+void res_destroy(PipeRes* r) {
+  if (!r) return;
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->upload) (void)hipStreamSynchronize(r->upload);
+  for (int s = 0; s < kSlots; s++)
+    if (r->slots[s].frames) (void)hipHostFree(r->slots[s].frames);
+  for (int k = 0; k < 2; k++) {
+    Outputs& x = r->outs[k];
+    if (r->d_in[k]) (void)hipFree(r->d_in[k]);
+    if (x.d_hashes) (void)hipFree(x.d_hashes);
+    if (x.d_quality) (void)hipFree(x.d_quality);
+    if (x.d_threshold) (void)hipFree(x.d_threshold);
+    if (x.d_pdqf) (void)hipFree(x.d_pdqf);
+    if (x.h_hashes) (void)hipHostFree(x.h_hashes);
+    if (x.h_quality) (void)hipHostFree(x.h_quality);
+    if (x.h_threshold) (void)hipHostFree(x.h_threshold);
+    if (x.h_pdqf) (void)hipHostFree(x.h_pdqf);
+    if (x.done) (void)hipEventDestroy(x.done);
+    if (x.uploaded) (void)hipEventDestroy(x.uploaded);
+  }
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  if (r->upload) (void)hipStreamDestroy(r->upload);
+  delete r;
+}
+
+hipError_t res_create(int device, uint32_t B, uint64_t frame_bytes, bool diag, PipeRes** out) {
+  PipeRes* r = new PipeRes();
+  r->device = device;
+  r->batch = B;
+  r->frame_bytes = frame_bytes;
+  r->diag = diag;
+  memset(r->stores, 0, sizeof r->stores);
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->upload, hipStreamNonBlocking);
+  for (int s = 0; s < kSlots && e == hipSuccess; s++) e = hipHostMalloc((void**)&r->slots[s].frames, B * frame_bytes, 0);
   for (int k = 0; k < 2 && e == hipSuccess; k++) {
-    Outputs& x = outs[k];
-    if (e == hipSuccess) e = hipMalloc(&d_in[k], B * frame_bytes);
+    Outputs& x = r->outs[k];
+    if (e == hipSuccess) e = hipMalloc(&r->d_in[k], B * frame_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&x.d_hashes, (size_t)B * 256);
     if (e == hipSuccess) e = hipMalloc((void**)&x.d_quality, (size_t)B * 32);
     if (e == hipSuccess) e = hipMalloc((void**)&x.d_threshold, (size_t)B * 32);
@@ -313,22 +348,94 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     if (e == hipSuccess && diag) e = hipHostMalloc((void**)&x.h_pdqf, (size_t)B * 8 * 256 * 4, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&x.uploaded, hipEventDisableTiming);
-    HostStores& r = stores[k];
-    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_hashes), &r.dst[0]);
-    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_quality), &r.dst[1]);
-    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_threshold), &r.dst[2]);
-    if (e == hipSuccess && diag) e = device_view(reinterpret_cast<uint4*>(x.h_pdqf), &r.dst[3]);
-    r.src[0] = reinterpret_cast<const uint4*>(x.d_hashes);
-    r.src[1] = reinterpret_cast<const uint4*>(x.d_quality);
-    r.src[2] = reinterpret_cast<const uint4*>(x.d_threshold);
-    r.src[3] = reinterpret_cast<const uint4*>(x.d_pdqf);
+    HostStores& st = r->stores[k];
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_hashes), &st.dst[0]);
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_quality), &st.dst[1]);
+    if (e == hipSuccess) e = device_view(reinterpret_cast<uint4*>(x.h_threshold), &st.dst[2]);
+    if (e == hipSuccess && diag) e = device_view(reinterpret_cast<uint4*>(x.h_pdqf), &st.dst[3]);
+    st.src[0] = reinterpret_cast<const uint4*>(x.d_hashes);
+    st.src[1] = reinterpret_cast<const uint4*>(x.d_quality);
+    st.src[2] = reinterpret_cast<const uint4*>(x.d_threshold);
+    st.src[3] = reinterpret_cast<const uint4*>(x.d_pdqf);
   }
+  if (e != hipSuccess) {
+    res_destroy(r);
+    r = nullptr;
+  }
+  *out = r;
+  return e;
+}
 
+// Idle resource sets of finished runs, the most recently used last.
+std::mutex g_res_mu;
+std::vector<PipeRes*> g_res_idle;
+constexpr size_t kIdleMax = 2;
+
+hipError_t res_acquire(uint32_t B, uint64_t frame_bytes, bool diag, PipeRes** out) {
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return e;
+  {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    for (size_t i = g_res_idle.size(); i-- > 0;) {
+      PipeRes* r = g_res_idle[i];
+      if (r->device == device && r->batch == B && r->frame_bytes == frame_bytes && r->diag == diag) {
+        g_res_idle.erase(g_res_idle.begin() + i);
+        *out = r;
+        return hipSuccess;
+      }
+    }
+  }
+  return res_create(device, B, frame_bytes, diag, out);
+}
+
+// Back to the cache after a run whose GPU work has completed.
+void res_release(PipeRes* r) {
+  std::vector<PipeRes*> evict;
+  {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    g_res_idle.push_back(r);
+    while (g_res_idle.size() > kIdleMax) {
+      evict.push_back(g_res_idle.front());
+      g_res_idle.erase(g_res_idle.begin());
+    }
+  }
+  for (PipeRes* x : evict) res_destroy(x);
+}
+
+}  // namespace
+
+int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
+  if (frames_out) *frames_out = 0;
+  const uint64_t frame_bytes = input_plane_bytes(o.input_kind);
+  if (frame_bytes == 0 || o.input_kind == kInputF32 || o.batch == 0) return -1;
+  const uint32_t B = o.batch;
+  const bool diag = o.format.type == OutputFormat::kDiagnostic;
+  auto fail_hip = [&](hipError_t err) {
+    if (error) *error = std::string("HIP: ") + hipGetErrorString(err);
+    return -2;
+  };
+  PipeRes* res = nullptr;
+  hipError_t e = res_acquire(B, frame_bytes, diag, &res);
+  if (e != hipSuccess) return fail_hip(e);
+  Slot* slots = res->slots;
+  for (int s = 0; s < kSlots; s++) {
+    slots[s].n = 0;
+    slots[s].status = 0;
+    slots[s].last = false;
+    slots[s].full = false;
+  }
+  Outputs* outs = res->outs;
+  void** d_in = res->d_in;
+  const HostStores* stores = res->stores;
+  const hipStream_t stream = res->stream, upload = res->upload;
+  bool hip_failed = false;
+  int rc = 0;
   std::mutex mu;
   std::condition_variable cv;
   bool stop_reader = false;
   std::thread reader;
-  if (e == hipSuccess) {
+  {
     reader = std::thread([&] {
       for (uint64_t k = 0;; k++) {
         Slot& s = slots[k % kSlots];
@@ -367,8 +474,6 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
         if (last) return;
       }
     });
-  } else {
-    rc = fail_hip(e);
   }
 
   uint64_t frames = 0;
@@ -391,7 +496,10 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     }
     cv.notify_all();
     pending = -1;
-    if (err != hipSuccess) return fail_hip(err);
+    if (err != hipSuccess) {
+      hip_failed = true;
+      return fail_hip(err);
+    }
     text.clear();
     format_frames(o.format, pending_n, o.dihedrals, x.h_hashes, x.h_quality, x.h_threshold, x.h_pdqf, text);
     if (!write_full(o.output_fd, text.data(), text.size())) {
@@ -449,6 +557,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
       }
       if (err == hipSuccess) err = hipEventRecord(x.done, stream);
       if (err != hipSuccess) {
+        hip_failed = true;
         rc = fail_hip(err);
         break;
       }
@@ -479,28 +588,23 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
   }
   cv.notify_all();
   if (reader.joinable()) reader.join();
-  (void)hipStreamSynchronize(stream);
-  if (upload) (void)hipStreamSynchronize(upload);
-  for (int s = 0; s < kSlots; s++)
-    if (slots[s].frames) (void)hipHostFree(slots[s].frames);
-  for (int k = 0; k < 2; k++) {
-    Outputs& x = outs[k];
-    (void)hipFree(d_in[k]);
-    (void)hipFree(x.d_hashes);
-    (void)hipFree(x.d_quality);
-    (void)hipFree(x.d_threshold);
-    (void)hipFree(x.d_pdqf);
-    if (x.h_hashes) (void)hipHostFree(x.h_hashes);
-    if (x.h_quality) (void)hipHostFree(x.h_quality);
-    if (x.h_threshold) (void)hipHostFree(x.h_threshold);
-    if (x.h_pdqf) (void)hipHostFree(x.h_pdqf);
-    if (x.done) (void)hipEventDestroy(x.done);
-    if (x.uploaded) (void)hipEventDestroy(x.uploaded);
-  }
-  if (stream) (void)hipStreamDestroy(stream);
-  if (upload) (void)hipStreamDestroy(upload);
+  hipError_t se = hipStreamSynchronize(stream);
+  if (se == hipSuccess) se = hipStreamSynchronize(upload);
+  // a set whose streams saw an error is not reused
+  if (hip_failed || se != hipSuccess) res_destroy(res);
+  else res_release(res);
   if (frames_out) *frames_out = frames;
   return rc;
+}
+
+size_t trim_pipe_cache() {
+  std::vector<PipeRes*> idle;
+  {
+    std::lock_guard<std::mutex> g(g_res_mu);
+    idle.swap(g_res_idle);
+  }
+  for (PipeRes* r : idle) res_destroy(r);
+  return idle.size();
 }
 
 }  // namespace ypdq

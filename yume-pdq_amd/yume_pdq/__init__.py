@@ -110,6 +110,8 @@ def lib():
         L.yume_pdq_amd_hash_ref_batch.argtypes = [vp, i, u64, u32, vp, u64, vp, vp, vp, vp, vp]
         L.yume_pdq_amd_pipe_run.restype = i
         L.yume_pdq_amd_pipe_run.argtypes = [C.POINTER(PipeConfig), C.POINTER(C.c_uint64)]
+        L.yume_pdq_amd_pipe_trim.restype = i
+        L.yume_pdq_amd_pipe_trim.argtypes = []
         L.yume_pdq_amd_format_frames.restype = C.c_int64
         L.yume_pdq_amd_format_frames.argtypes = [C.c_char_p, u32, i, vp, vp, vp, vp, vp, u64]
         L.yume_pdq_amd_format_bound.restype = u64
@@ -949,6 +951,12 @@ def pipe(source, dest, fmt="bin", input_format="luma8", dihedrals=False, batch=6
     finally:
         for fd in own:
             os.close(fd)
+
+
+def pipe_trim():
+    """Free the buffer sets finished pipe runs keep for the next run of the
+    same shape; returns how many were freed."""
+    return lib().yume_pdq_amd_pipe_trim()
 
 
 def hash_ref_batch(planes, want_pdqf=False, stream=None):
