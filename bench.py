@@ -798,10 +798,12 @@ def pipe_section(args, yume_pdq, torch, planes_u8):
         log(f"[bench]   pipe {out}")
         return {"frames": frames, "input": "luma8 512x512 frames from a file in the page cache",
                 "output": "/dev/null", "rates": out,
-                "note": ("one yume_pdq_amd_pipe_run call per line, setup (pinned and device buffers) "
-                         "included; the reference's pipe hashes on two CPU threads")}
+                "note": ("one yume_pdq_amd_pipe_run call per line, after a warm-up call of the same shape "
+                         "whose page-locked ring and device buffers the timed call reuses; the reference's "
+                         "pipe hashes on two CPU threads")}
     finally:
         os.unlink(path)
+        yume_pdq.pipe_trim()
 
 
 def drop_in_ffi_us(yume_pdq, image, n=300):

@@ -408,8 +408,17 @@ int yume_pdq_amd_batcher_buffers(const YumePdqAmdBatcher *b, YumePdqAmdBatchBuff
 int yume_pdq_amd_batcher_run(YumePdqAmdBatcher *b, uint32_t n_images, int use_graph);
 /* Time the vector-ALU matcher and the matrix-core matcher at 2/4/8/16 query
  * tiles per wave for query counts 8, 16, 32, ... 8 max_images on the set's
- * shards, and use the fastest per bucket from then on. */
+ * shards, and keep the fastest per bucket in the process-wide tuning cache:
+ * every batcher and server context on a shard set of the same shape (its
+ * shards' row counts, in order), GPU and threshold uses them from then on. */
 int yume_pdq_amd_batcher_autotune(YumePdqAmdBatcher *b, int iterations);
+/* The tuning cache as a text file, to carry choices to another process:
+ * save returns the entries written, load the entries read and merged (a
+ * malformed file loads nothing: EINVAL), clear the entries dropped.  No
+ * counterpart in the reference. */
+int yume_pdq_amd_tune_save(const char *path);
+int yume_pdq_amd_tune_load(const char *path);
+int yume_pdq_amd_tune_clear(void);
 int yume_pdq_amd_batcher_tune_records(const YumePdqAmdBatcher *b, YumePdqAmdTuneRecord *out,
                                       uint32_t cap, uint32_t *n);
 int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher *b, uint32_t n_queries, uint32_t *path,
@@ -442,6 +451,8 @@ typedef struct YumePdqAmdServerConfig {
   int use_graph;
   int busy_poll;              /* worker spins (yielding) while batches are in flight */
   int in_order_compute;       /* batches' kernels run one batch after another (uploads overlap) */
+  uint32_t autotune_iterations; /* > 0: autotune at start unless choices for the shard set's
+                                   shape are cached (yume_pdq_amd_batcher_autotune, _tune_load) */
 } YumePdqAmdServerConfig;
 typedef struct YumePdqAmdResult {
   int status;                 /* YUME_PDQ_AMD_OK or an error of the batch */

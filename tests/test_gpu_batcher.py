@@ -164,6 +164,64 @@ def test_batcher_autotune(gpu, dev, shard_set):
         b.close()
 
 
+def test_tuning_shared_and_saved(gpu, dev, shard_set, tmp_path):
+    """Autotuner choices are kept per (GPU, threshold, shard-set shape) for
+    the whole process: a second batcher on the same set uses them without
+    tuning, clearing the cache brings back the defaults, a saved file brings
+    the choices back, and another threshold is another key."""
+    buckets = (8, 16, 32, 64, 128)
+    gpu.tune_clear()
+    b1 = gpu.Batcher(shard_set, 16, threshold=31)
+    b2 = b3 = None
+    try:
+        defaults = {nq: b1.choice(nq) for nq in buckets}
+        say("[batcher] autotune for the cache")
+        b1.autotune(iterations=2)
+        tuned = {nq: b1.choice(nq) for nq in buckets}
+        path = tmp_path / "tune.txt"
+        assert gpu.tune_save(str(path)) == 1
+        b2 = gpu.Batcher(shard_set, 16, threshold=31)
+        assert {nq: b2.choice(nq) for nq in buckets} == tuned
+        assert gpu.tune_clear() == 1
+        assert {nq: b2.choice(nq) for nq in buckets} == defaults
+        assert gpu.tune_load(str(path)) == 1
+        assert {nq: b2.choice(nq) for nq in buckets} == tuned
+        assert {nq: b1.choice(nq) for nq in buckets} == tuned
+        b3 = gpu.Batcher(shard_set, 16, threshold=20)
+        assert {nq: b3.choice(nq) for nq in buckets} == defaults
+    finally:
+        for b in (b1, b2, b3):
+            if b is not None:
+                b.close()
+        gpu.tune_clear()
+
+
+def test_server_autotunes_once(gpu, dev, db_parts, shard_set):
+    """A server started with autotune_iterations tunes when the cache has no
+    choices for its shard set and stores them for everyone; its results equal
+    a batcher's."""
+    import torch
+    gpu.tune_clear()
+    imgs = synthetic.images_u8(6, start=40)
+    srv = gpu.Server(shard_set, max_batch_images=16, hbm_budget_bytes=4 << 30, max_contexts=2,
+                     autotune_iterations=1)
+    try:
+        rid = srv.submit(imgs)
+        r = srv.wait(rid, timeout_ms=60_000)
+    finally:
+        srv.close()
+    assert r["status"] == 0
+    b = gpu.Batcher(shard_set, 16, threshold=31)
+    try:
+        h, q, t, total, hits = b.run(torch.from_numpy(imgs).to(dev))
+    finally:
+        b.close()
+        saved = gpu.tune_clear()
+    assert saved == 1
+    assert np.array_equal(np.asarray(r["hashes"]).reshape(-1), h.cpu().numpy().reshape(-1))
+    assert sorted(map(tuple, triples(r["hits"]))) == sorted(map(tuple, triples(hits)))
+
+
 def test_shards_load_raw(gpu, dev, tmp_path):
     """A raw N x 32-byte file split into shards of 3000 rows equals one
     database of the same rows."""

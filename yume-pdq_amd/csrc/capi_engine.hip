@@ -180,6 +180,20 @@ int yume_pdq_amd_batcher_choice(const YumePdqAmdBatcher* b, uint32_t n_queries, 
   return YUME_PDQ_AMD_OK;
 }
 
+int yume_pdq_amd_tune_save(const char* path) {
+  if (!path) return YUME_PDQ_AMD_EINVAL;
+  const int n = ypdq::tune_save(path);
+  return n < 0 ? YUME_PDQ_AMD_EIO : n;
+}
+
+int yume_pdq_amd_tune_load(const char* path) {
+  if (!path) return YUME_PDQ_AMD_EINVAL;
+  const int n = ypdq::tune_load(path);
+  return n < 0 ? YUME_PDQ_AMD_EINVAL : n;
+}
+
+int yume_pdq_amd_tune_clear(void) { return (int)ypdq::tune_clear(); }
+
 int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig* cfg, YumePdqAmdServer** out) {
   if (!s || !cfg || !out) return YUME_PDQ_AMD_EINVAL;
   YumePdqAmdServer* v = new (std::nothrow) YumePdqAmdServer();
@@ -194,6 +208,7 @@ int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig
   c.use_graph = cfg->use_graph != 0;
   c.busy_poll = cfg->busy_poll != 0;
   c.in_order_compute = cfg->in_order_compute != 0;
+  c.autotune_iterations = cfg->autotune_iterations;
   const hipError_t e = v->server.start(s->set, c);
   if (e != hipSuccess) {
     delete v;

@@ -8,6 +8,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "pdq_device.h"
@@ -28,6 +30,37 @@ struct TuneRecord {
   MatcherChoice choice;
   float ms;
 };
+
+// Autotuner results shared by every batch context of the process: the
+// matcher choice per query-count bucket (first query count of the bucket ->
+// choice), keyed by the GPU (architecture name and compute units), the
+// threshold and the shard set's shape (its shards' row counts, in order).
+struct TuneKey {
+  std::string arch;
+  int cus = 0;
+  uint32_t threshold = 0;
+  std::vector<uint64_t> shard_rows;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(arch, cus, threshold, shard_rows) < std::tie(o.arch, o.cus, o.threshold, o.shard_rows);
+  }
+  bool operator==(const TuneKey& o) const {
+    return arch == o.arch && cus == o.cus && threshold == o.threshold && shard_rows == o.shard_rows;
+  }
+};
+using TuneChoices = std::map<uint32_t, MatcherChoice>;
+
+// The cached choices of `k` into *out (false if none).
+bool tune_lookup(const TuneKey& k, TuneChoices* out);
+// Adds (replacing bucket by bucket) the choices of `k`.
+void tune_store(const TuneKey& k, const TuneChoices& c);
+// Bumped by every store, load and clear: contexts re-read the cache when it moves.
+uint64_t tune_generation();
+size_t tune_clear();
+// Text file: one line per key, "arch cus threshold n_shards rows... :
+// bucket=valu|mfma:qt ...".  save returns the lines written, load the lines
+// read (merged into the cache), -1 on an I/O or format error.
+int tune_save(const char* path);
+int tune_load(const char* path);
 
 class ShardSet {
  public:
@@ -60,6 +93,8 @@ class ShardSet {
                           hipStream_t stream, void* record_area = nullptr) const;
 
   uint32_t count() const { return (uint32_t)shards_.size(); }
+  // The set's autotuning key at `threshold` on the current device.
+  TuneKey tune_key(uint32_t threshold) const;
   uint64_t total_rows() const;
   uint64_t device_bytes() const;
   uint64_t version() const { return version_; }
@@ -99,8 +134,14 @@ class BatchContext {
   // for this context's previous batch first unless the caller knows it is
   // done (wait_previous = false).
   hipError_t run(uint32_t n_images, bool use_graph, bool wait_previous = true);
+  // Times the matchers per query-count bucket and stores the fastest in the
+  // process-wide tuning cache, under the shard set's current shape.
   hipError_t autotune(int iterations, std::vector<TuneRecord>* records);
+  // The matcher for n_queries: the cached choice for the shard set's current
+  // shape if there is one, else a default.
   MatcherChoice choice_for(uint32_t n_queries) const;
+  // True if the tuning cache holds choices for the shard set's current shape.
+  bool tuned() const;
   // Instantiate now the graph of every matcher choice that some batch size
   // up to max_images() selects, so that no batch pays for it later.
   hipError_t prepare_graphs();
@@ -132,6 +173,7 @@ class BatchContext {
   };
   hipError_t enqueue_body(const MatcherChoice& c, hipStream_t s);
   hipError_t graph_for(const MatcherChoice& c, hipGraphExec_t* exec);
+  void refresh_choices() const;
 
   std::shared_ptr<ShardSet> shards_;
   uint32_t max_images_ = 0;
@@ -151,7 +193,10 @@ class BatchContext {
   uint32_t* d_sizes_ = nullptr;  // device {n_images, n_queries}
   uint32_t* h_sizes_ = nullptr;  // pinned staging of the same
   void* d_records_ = nullptr;    // the shared-rows matcher's candidate records
-  std::map<uint32_t, MatcherChoice> choices_;  // first query count of a bucket -> choice
+  // the cache's choices for key_, read at tune generation gen_
+  mutable TuneChoices choices_;
+  mutable TuneKey key_;
+  mutable uint64_t gen_ = ~0ull;
   std::map<int, Graph> graphs_;
 };
 

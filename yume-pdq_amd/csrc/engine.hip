@@ -17,7 +17,10 @@
 //                 graph after one 8-byte upload of its size.
 //   Autotuner     per query-count bucket, times the vector-ALU kernel and the
 //                 matrix-core kernel at 2/4/8/16 query tiles per wave on the
-//                 context's own shards and keeps the fastest.
+//                 context's own shards and keeps the fastest in a cache that
+//                 every context of the process reads, keyed by GPU,
+//                 threshold and shard-set shape; the cache can be saved to
+//                 and loaded from a text file.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -25,7 +28,10 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -56,6 +62,115 @@ __global__ void replace_rows_kernel(uint4* __restrict__ rows, uint4* __restrict_
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Tuning cache
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoices> g_tune;
+std::atomic<uint64_t> g_tune_gen{0};
+
+std::string choice_text(const MatcherChoice& c) { return c.mfma ? "mfma:" + std::to_string(c.qt) : "valu"; }
+
+bool parse_choice(const std::string& t, MatcherChoice* c) {
+  if (t == "valu") {
+    c->mfma = false;
+    return true;
+  }
+  if (t.rfind("mfma:", 0) != 0) return false;
+  const int qt = atoi(t.c_str() + 5);
+  if (qt != 2 && qt != 4 && qt != 8 && qt != 16) return false;
+  c->mfma = true;
+  c->qt = qt;
+  return true;
+}
+}  // namespace
+
+bool tune_lookup(const TuneKey& k, TuneChoices* out) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+void tune_store(const TuneKey& k, const TuneChoices& c) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  TuneChoices& d = g_tune[k];
+  for (const auto& kv : c) d[kv.first] = kv.second;
+  g_tune_gen++;
+}
+
+uint64_t tune_generation() { return g_tune_gen.load(); }
+
+size_t tune_clear() {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  const size_t n = g_tune.size();
+  g_tune.clear();
+  g_tune_gen++;
+  return n;
+}
+
+int tune_save(const char* path) {
+  if (!path) return -1;
+  std::ostringstream o;
+  int lines = 0;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    o << "# yume_pdq_amd matcher tuning: arch cus threshold n_shards rows... : bucket=choice ...\n";
+    for (const auto& kv : g_tune) {
+      const TuneKey& k = kv.first;
+      o << k.arch << ' ' << k.cus << ' ' << k.threshold << ' ' << k.shard_rows.size();
+      for (uint64_t r : k.shard_rows) o << ' ' << r;
+      o << " :";
+      for (const auto& b : kv.second) o << ' ' << b.first << '=' << choice_text(b.second);
+      o << '\n';
+      lines++;
+    }
+  }
+  std::ofstream f(path, std::ios::trunc);
+  if (!f) return -1;
+  f << o.str();
+  f.close();
+  return f ? lines : -1;
+}
+
+int tune_load(const char* path) {
+  if (!path) return -1;
+  std::ifstream f(path);
+  if (!f) return -1;
+  std::vector<std::pair<TuneKey, TuneChoices>> got;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream in(line);
+    TuneKey k;
+    size_t n = 0;
+    if (!(in >> k.arch >> k.cus >> k.threshold >> n) || n > 4096) return -1;
+    k.shard_rows.resize(n);
+    for (size_t i = 0; i < n; i++)
+      if (!(in >> k.shard_rows[i])) return -1;
+    std::string colon;
+    if (!(in >> colon) || colon != ":") return -1;
+    TuneChoices c;
+    std::string item;
+    while (in >> item) {
+      const size_t eq = item.find('=');
+      if (eq == std::string::npos || eq == 0) return -1;
+      char* end = nullptr;
+      const unsigned long nq = strtoul(item.c_str(), &end, 10);
+      if (end != item.c_str() + eq || nq == 0 || nq > (1ul << 20)) return -1;
+      MatcherChoice mc;
+      if (!parse_choice(item.substr(eq + 1), &mc)) return -1;
+      c[(uint32_t)nq] = mc;
+    }
+    if (c.empty()) return -1;
+    got.emplace_back(std::move(k), std::move(c));
+  }
+  for (const auto& kc : got) tune_store(kc.first, kc.second);
+  return (int)got.size();
+}
 
 // ---------------------------------------------------------------------------
 // ShardSet
@@ -167,6 +282,30 @@ hipError_t ShardSet::rebuild_views_locked() {
   return hipMemcpy(d_views_, v.data(), v.size() * sizeof(ShardView), hipMemcpyHostToDevice);
 }
 
+TuneKey ShardSet::tune_key(uint32_t threshold) const {
+  static std::mutex mu;
+  static std::map<int, std::string> arch;  // device -> architecture name
+  TuneKey k;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = arch.find(dev);
+    if (it == arch.end()) {
+      hipDeviceProp_t p;
+      std::string a = hipGetDeviceProperties(&p, dev) == hipSuccess ? std::string(p.gcnArchName) : "unknown";
+      std::replace(a.begin(), a.end(), ' ', '_');
+      if (a.empty()) a = "unknown";
+      it = arch.emplace(dev, a).first;
+    }
+    k.arch = it->second;
+  }
+  k.cus = cus_;
+  k.threshold = threshold;
+  for (auto& s : shards_) k.shard_rows.push_back(s.n_rows);
+  return k;
+}
+
 uint64_t ShardSet::total_rows() const {
   uint64_t t = 0;
   for (auto& s : shards_) t += s.n_rows;
@@ -255,9 +394,10 @@ hipError_t BatchContext::init(std::shared_ptr<ShardSet> shards, uint32_t max_ima
   if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_sizes_), 2 * sizeof(uint32_t), hipHostMallocDefault)) !=
       hipSuccess)
     return e;
-  // default choices before any autotuning: vector ALU below 32 queries, eight
-  // query tiles per wave on the matrix cores above
+  // choices come from the tuning cache (refresh_choices); without an entry
+  // for the shard set's shape, choice_for's defaults apply
   choices_.clear();
+  gen_ = ~0ull;
   return hipSuccess;
 }
 
@@ -281,7 +421,25 @@ static bool shared_pays(const MatcherChoice& c, uint32_t nq) {
   return c.mfma && (nq + c.qt * 16 - 1) / (c.qt * 16) >= 4;
 }
 
+void BatchContext::refresh_choices() const {
+  const uint64_t gen = tune_generation();
+  TuneKey k = shards_->tune_key(threshold_);
+  if (gen == gen_ && k == key_) return;
+  choices_.clear();
+  (void)tune_lookup(k, &choices_);
+  key_ = std::move(k);
+  gen_ = gen;
+}
+
+bool BatchContext::tuned() const {
+  refresh_choices();
+  return !choices_.empty();
+}
+
 MatcherChoice BatchContext::choice_for(uint32_t nq) const {
+  refresh_choices();
+  // without tuning: vector ALU below 32 queries, eight query tiles per wave
+  // on the matrix cores from 32
   auto it = choices_.upper_bound(nq);
   MatcherChoice c;
   if (it != choices_.begin()) {
@@ -377,7 +535,7 @@ hipError_t BatchContext::run(uint32_t n_images, bool use_graph, bool wait_previo
 
 hipError_t BatchContext::autotune(int iterations, std::vector<TuneRecord>* records) {
   if (iterations < 1) iterations = 1;
-  choices_.clear();
+  TuneChoices tuned;
   hipEvent_t a, b;
   hipError_t e;
   if ((e = hipEventCreate(&a)) != hipSuccess) return e;
@@ -429,10 +587,12 @@ hipError_t BatchContext::autotune(int iterations, std::vector<TuneRecord>* recor
         best_c = c;
       }
     }
-    choices_[nq] = best_c;
+    tuned[nq] = best_c;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  tune_store(shards_->tune_key(threshold_), tuned);
+  gen_ = ~0ull;
   return hipSuccess;
 }
 

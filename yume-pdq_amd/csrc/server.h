@@ -34,6 +34,10 @@ struct ServerConfig {
   // computes, and starts when it ends (or when it is full).  Concurrent
   // batches would share the GPU and all finish late.
   bool in_order_compute = true;
+  // With no tuned choices cached for the shard set's shape, the first
+  // context runs the autotuner with this many timed launches per candidate
+  // at start (0: use the cached choices or the defaults).
+  uint32_t autotune_iterations = 0;
 };
 
 struct RequestResult {

@@ -73,6 +73,11 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
       return e;
     // admission by HBM budget: a context is kept only if it fits
     if (device_bytes() + s->ctx->device_bytes() > cfg.hbm_budget_bytes) break;
+    // tuned choices are shared through the process-wide cache, so the first
+    // context tunes for all (before any graph is built on a choice)
+    if (k == 0 && cfg.autotune_iterations > 0 && !s->ctx->tuned() &&
+        (e = s->ctx->autotune((int)cfg.autotune_iterations, nullptr)) != hipSuccess)
+      return e;
     // every graph a batch size can select is built before the first request
     if (cfg.use_graph && (e = s->ctx->prepare_graphs()) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming)) != hipSuccess) return e;

@@ -54,7 +54,8 @@ class BatchBuffers(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch_images", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("max_contexts", C.c_uint32),
                 ("threshold", C.c_uint32), ("dihedral_mode", C.c_uint32), ("hit_capacity", C.c_uint64),
-                ("use_graph", C.c_int), ("busy_poll", C.c_int), ("in_order_compute", C.c_int)]
+                ("use_graph", C.c_int), ("busy_poll", C.c_int), ("in_order_compute", C.c_int),
+                ("autotune_iterations", C.c_uint32)]
 
 
 class ServerResult(C.Structure):
@@ -157,6 +158,9 @@ def lib():
                 ("yume_pdq_amd_batcher_tune_records", i, [vp, C.POINTER(TuneRecord), u32,
                                                           C.POINTER(C.c_uint32)]),
                 ("yume_pdq_amd_batcher_choice", i, [vp, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+                ("yume_pdq_amd_tune_save", i, [C.c_char_p]),
+                ("yume_pdq_amd_tune_load", i, [C.c_char_p]),
+                ("yume_pdq_amd_tune_clear", i, []),
                 ("yume_pdq_amd_server_create", i, [vp, C.POINTER(ServerConfig), pp]),
                 ("yume_pdq_amd_server_destroy", None, [vp]),
                 ("yume_pdq_amd_server_submit", i, [vp, vp, u32, C.POINTER(C.c_uint64)]),
@@ -814,6 +818,29 @@ class Batcher:
         return ("mfma" if path.value == MATCH_MFMA else "valu"), qt.value
 
 
+def tune_save(path):
+    """Write the process-wide matcher tuning cache (autotuner choices per GPU,
+    threshold and shard-set shape) to a text file; returns the entries."""
+    n = lib().yume_pdq_amd_tune_save(os.fsencode(path))
+    if n < 0:
+        raise YumePdqError(f"tune_save: {lib().yume_pdq_amd_strerror(n).decode()} ({n})")
+    return n
+
+
+def tune_load(path):
+    """Merge a file written by tune_save into the tuning cache; returns the
+    entries read (a malformed file loads nothing and raises)."""
+    n = lib().yume_pdq_amd_tune_load(os.fsencode(path))
+    if n < 0:
+        raise YumePdqError(f"tune_load: {lib().yume_pdq_amd_strerror(n).decode()} ({n})")
+    return n
+
+
+def tune_clear():
+    """Drop every cached tuning entry; returns how many there were."""
+    return lib().yume_pdq_amd_tune_clear()
+
+
 class Server:
     """Continuous-batching server over a shard set: submit() u8 planes from
     any thread, wait() for each request's hashes, qualities, thresholds and
@@ -821,10 +848,10 @@ class Server:
 
     def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
                  dihedral_mode=DIHEDRAL_AVX2, hit_capacity=1 << 20, use_graph=True, busy_poll=True,
-                 in_order_compute=True):
+                 in_order_compute=True, autotune_iterations=0):
         cfg = ServerConfig(int(max_batch_images), int(hbm_budget_bytes), int(max_contexts), int(threshold),
                            int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0, 1 if busy_poll else 0,
-                           1 if in_order_compute else 0)
+                           1 if in_order_compute else 0, int(autotune_iterations))
         h = C.c_void_p()
         _check(lib().yume_pdq_amd_server_create(shards._h, C.byref(cfg), C.byref(h)), "server_create")
         self._h = h
