@@ -196,6 +196,38 @@ def test_tuning_shared_and_saved(gpu, dev, shard_set, tmp_path):
         gpu.tune_clear()
 
 
+def test_query_tiles_follow_padding(gpu, dev, shard_set, tmp_path):
+    """A bucket tuned to 16 query tiles per wave (256 queries per group) at
+    its first query count runs 320 queries at 8 tiles (384 computed instead of
+    512) and keeps 16 where the groups are full; the hits do not change."""
+    import torch
+    gpu.tune_clear()
+    b = gpu.Batcher(shard_set, 64, threshold=31)
+    try:
+        b.autotune(iterations=1)
+        path = tmp_path / "tune.txt"
+        assert gpu.tune_save(str(path)) == 1
+        line = [x for x in path.read_text().splitlines() if not x.startswith("#")][0]
+        key = line.split(" :")[0]
+        path.write_text(f"{key} : 8=valu 32=mfma:2 256=mfma:16\n")
+        gpu.tune_clear()
+        assert gpu.tune_load(str(path)) == 1
+        assert b.choice(256) == ("mfma", 16)
+        assert b.choice(512) == ("mfma", 16)
+        assert b.choice(320) == ("mfma", 8)
+        assert b.choice(264) == ("mfma", 8)
+        assert b.choice(40) == ("mfma", 2)
+        imgs = torch.from_numpy(synthetic.images_u8(40, start=70)).to(dev)
+        got = b.run(imgs)
+        gpu.tune_clear()
+        ref = b.run(imgs)
+        assert got[3] == ref[3]
+        assert sorted(map(tuple, triples(got[4]))) == sorted(map(tuple, triples(ref[4])))
+    finally:
+        b.close()
+        gpu.tune_clear()
+
+
 def test_server_autotunes_once(gpu, dev, db_parts, shard_set):
     """A server started with autotune_iterations tunes when the cache has no
     choices for its shard set and stores them for everyone; its results equal

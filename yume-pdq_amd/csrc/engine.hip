@@ -436,6 +436,17 @@ bool BatchContext::tuned() const {
   return !choices_.empty();
 }
 
+// Relative cost per query of the matrix-core matcher at qt query tiles per
+// wave (a wave's row loads and expansions are shared by its 16 qt queries):
+// 10^8 rows at 512 queries take 3.75 / 2.52 / 2.04 / 2.05 ms at 2 / 4 / 8 / 16.
+static double mfma_query_cost(int qt) { return qt >= 8 ? 1.0 : qt == 4 ? 1.25 : 1.8; }
+
+// Queries the matrix-core matcher computes for nq: whole groups of 16 qt.
+static uint64_t padded_queries(uint32_t nq, int qt) {
+  const uint64_t g = (uint64_t)qt * 16;
+  return (nq + g - 1) / g * g;
+}
+
 MatcherChoice BatchContext::choice_for(uint32_t nq) const {
   refresh_choices();
   // without tuning: vector ALU below 32 queries, eight query tiles per wave
@@ -448,6 +459,22 @@ MatcherChoice BatchContext::choice_for(uint32_t nq) const {
   } else {
     c.mfma = threshold_ < 64 && nq >= 32;
     c.qt = 8;
+  }
+  if (c.mfma) {
+    // a bucket is tuned at its first query count, where every candidate's
+    // groups are full; above it, the last group of a wide tile can be mostly
+    // padding (nq = 320 at 16 tiles computes 512 queries).  Take the narrower
+    // tile when its padded work costs less.
+    int best = c.qt;
+    double best_cost = (double)padded_queries(nq, c.qt) * mfma_query_cost(c.qt);
+    for (int qt = c.qt / 2; qt >= 2; qt /= 2) {
+      const double cost = (double)padded_queries(nq, qt) * mfma_query_cost(qt);
+      if (cost < best_cost) {
+        best = qt;
+        best_cost = cost;
+      }
+    }
+    c.qt = best;
   }
   c.shared = shared_pays(c, nq);
   return c;
