@@ -517,12 +517,12 @@ hipError_t launch_block_waves(const void* planes, int input_kind, uint64_t plane
 }  // namespace
 
 int hash_layout_for(uint32_t n_images, int dct) {
-  // Exact arithmetic: eight waves per image (about 40 us for any batch up to
-  // 256 images, against 90 us for a wave per image) until the batch is large
-  // enough that a wave per image fills the GPU with less LDS per image (from
-  // about 1024 images on 256 CUs).  Matrix-core arithmetic exists only in the
-  // block form.
-  return (dct == kDctExact && n_images >= 1024) ? 1 : 8;
+  // Exact arithmetic: eight waves per image (18 us for one u8 image, 23 us
+  // for 128, 71 us for 1024, against 60 / 71 / 78 us for a wave per image)
+  // until the batch is large enough that a wave per image fills the GPU with
+  // less LDS per image (1536 u8 images: 91 us against 105 us; 4096: 218
+  // against 261 us).  Matrix-core arithmetic exists only in the block form.
+  return (dct == kDctExact && n_images > 1024) ? 1 : 8;
 }
 
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
