@@ -228,10 +228,11 @@ int yume_pdq_amd_match_with_plane(const uint8_t *d_db, const void *d_plane, uint
  * library's own device memory (2^20 records of 16 bytes + 256 bytes, held
  * from the library's load until the process ends) instead of an area
  * allocated per stream.  Limit: a call's worst case -- one record per
- * 128-query group and 1024 database rows, i.e.
- * ceil(ceil(n_rows / 1024) / 8) * 8 * ceil(n_queries / 512) * 4 records --
- * must fit the area (at 1024 queries: up to 134M rows; at 10M rows: up to
- * 13,312 queries); a larger call returns YUME_PDQ_AMD_ESCRATCH and enqueues
+ * query group of g queries and 1024 database rows (g = 256 from 1009
+ * queries up, 128 below), i.e.
+ * ceil(ceil(n_rows / 1024) / 8) * 8 * ceil(n_queries / (4 g)) * 4 records --
+ * must fit the area (at 1024 queries: up to 268M rows; at 10M rows: up to
+ * 26,624 queries); a larger call returns YUME_PDQ_AMD_ESCRATCH and enqueues
  * nothing.  Every call on one GPU shares the area, which keeps the last
  * call's records (the call clears the count it starts from): calls on
  * different streams of one GPU that overlap in time race and must be

@@ -127,24 +127,25 @@ def test_append_flag(gpu, dev, data):
 
 
 def test_too_large_is_refused_before_enqueueing(gpu, dev):
-    """10M rows x 16,384 queries needs up to 1,250,304 records, more than
-    the 2^20 the area holds: ESCRATCH, and the counter is left untouched
-    (nothing ran).  13,312 queries fit and run."""
+    """10M rows x 32,768 queries needs up to 1,250,304 records (one per
+    1024 rows and 256-query group), more than the 2^20 the area holds:
+    ESCRATCH, and the counter is left untouched (nothing ran).  26,624
+    queries (1,015,872 records) fit and run."""
     import torch
     d_db = torch.randint(0, 256, (10_000_000, 32), dtype=torch.uint8, device=dev)
-    d_q = torch.randint(0, 256, (16384, 32), dtype=torch.uint8, device=dev)
+    d_q = torch.randint(0, 256, (32768, 32), dtype=torch.uint8, device=dev)
     plane = gpu.build_prefix_plane(d_db)
     h, c = buffers(dev, cap=1 << 10)
-    say("[noalloc] 10M x 16384: refused")
+    say("[noalloc] 10M x 32768: refused")
     assert call(gpu, "noalloc", d_db, plane, d_q, 31, h, c) == ESCRATCH
     torch.cuda.synchronize()
     assert int(c.item()) == 12345
     assert gpu.lib().yume_pdq_amd_strerror(ESCRATCH).decode().startswith("the call needs more")
-    say("[noalloc] 10M x 13312: runs")
-    assert call(gpu, "noalloc", d_db, plane, d_q[:13312], 31, h, c) == 0
+    say("[noalloc] 10M x 26624: runs")
+    assert call(gpu, "noalloc", d_db, plane, d_q[:26624], 31, h, c) == 0
     n1, _ = result(h, c)
     h2, c2 = buffers(dev, cap=1 << 10)
-    assert call(gpu, "alloc", d_db, plane, d_q[:13312], 31, h2, c2) == 0
+    assert call(gpu, "alloc", d_db, plane, d_q[:26624], 31, h2, c2) == 0
     n2, _ = result(h2, c2)
     assert n1 == n2
 

@@ -220,7 +220,21 @@ __device__ __forceinline__ void fold_flags(const v4f& v, uint32_t& m) {
   asm volatile("" : "+v"(m));
 }
 
-// Pair P (row tile P / (QT/2), query tiles 2 (P % (QT/2)) and +1): its first
+// The query pair of pair P of a row tile (of QT / 2 pairs).  Pair P's
+// products go to result register P % 4 and are folded into flag chain P % 4;
+// at QT = 16 the pairs are ordered so that chains 0 and 1 hold query tiles
+// 0-7 and chains 2 and 3 tiles 8-15, so one vote per half tells which half
+// of the query tiles has candidates.
+template <int kHalf>
+__device__ constexpr int query_pair(int p) {
+  if constexpr (kHalf == 8) {
+    constexpr int order[8] = {0, 1, 4, 5, 2, 3, 6, 7};
+    return order[p % 8];
+  }
+  return p % kHalf;
+}
+
+// Pair P (row tile P / (QT/2), query tiles 2 query_pair(P) and +1): its first
 // product issues at step P into result register P % 4, its second at step
 // P + 1 (one product after, so the accumulation does not wait), and its bits
 // are folded at step P + 4, before the register takes pair P + 4.  The four
@@ -235,9 +249,9 @@ __device__ __forceinline__ void group_flags_step(const Expanded (&qa)[QT], const
   v4f& cur = (S % 4 == 0) ? r0 : (S % 4 == 1) ? r1 : (S % 4 == 2) ? r2 : r3;
   v4f& prev = (S % 4 == 1) ? r0 : (S % 4 == 2) ? r1 : (S % 4 == 3) ? r2 : r3;
   if constexpr (S >= 4) fold_flags(cur, acc[(S - 4) % kFlagChains]);
-  if constexpr (S < kN) cur = mfma_acc<kScale1>(qa[2 * (S % kHalf)], rb[S / kHalf], c0);
+  if constexpr (S < kN) cur = mfma_acc<kScale1>(qa[2 * query_pair<kHalf>(S)], rb[S / kHalf], c0);
   if constexpr (S >= 1 && S - 1 < kN)
-    prev = mfma_acc<kScale512>(qa[2 * ((S - 1) % kHalf) + 1], rb[(S - 1) / kHalf], prev);
+    prev = mfma_acc<kScale512>(qa[2 * query_pair<kHalf>(S - 1) + 1], rb[(S - 1) / kHalf], prev);
   // keep this order: hoisting the products runs out of registers
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (S + 1 < kN + 4) group_flags_step<QT, kTiles, S + 1>(qa, rb, c0, acc, r0, r1, r2, r3);
@@ -253,6 +267,15 @@ __device__ __forceinline__ void group_flags(const Expanded (&qa)[QT], const Expa
 
 __device__ __forceinline__ bool flags_set(const uint32_t (&acc)[kFlagChains]) {
   return ((acc[0] | acc[1] | acc[2] | acc[3]) & kFlagMask) != 0;
+}
+
+// Flags of query tiles 0-7 (chains 0 and 1) and 8-15 (chains 2 and 3) at
+// QT = 16.
+__device__ __forceinline__ bool flags_set_low(const uint32_t (&acc)[kFlagChains]) {
+  return ((acc[0] | acc[1]) & kFlagMask) != 0;
+}
+__device__ __forceinline__ bool flags_set_high(const uint32_t (&acc)[kFlagChains]) {
+  return ((acc[2] | acc[3]) & kFlagMask) != 0;
 }
 
 // The rare path: recompute one tile's products and re-check every pair that
@@ -429,6 +452,10 @@ constexpr int kStagePerWave = 1;                          // tiles a wave stages
 constexpr int kGroupTiles = kMfmaWaves * kStagePerWave;  // tiles per barrier and per vote
 static_assert(kSharedTiles % kGroupTiles == 0 && kSharedTiles / kGroupTiles <= 32,
               "one bit of a 32-bit record mask per group of tiles");
+// At QT = 16 a record's mask holds query tiles 0-7's groups in its low bits
+// and tiles 8-15's from bit kHalfShift.
+constexpr int kHalfShift = 16;
+static_assert(kSharedTiles / kGroupTiles <= kHalfShift, "two halves in a record mask");
 static_assert((uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 % kSharedRows == 0, "sub-chunks of a chunk");
 constexpr uint64_t kSubsPerChunk = (uint64_t)kMfmaWaves * kMfmaTilesPerWave * 16 / kSharedRows;
 
@@ -537,9 +564,15 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
         ring[a][st] = word(g + 1 + kRing < ngroups ? g + 1 + kRing : ngroups - 1, st);
       }
       // the vote is a wave-uniform mask, so the pending bits stay in scalar
-      // registers
-      const uint64_t vote = __ballot(flags_set(acc));
-      pending |= (uint64_t)(vote != 0) << g;
+      // registers; at QT = 16 one vote per half of the query tiles, the
+      // second half's bits kHalfShift above the first's
+      if constexpr (QT == 16) {
+        const uint64_t lo = __ballot(flags_set_low(acc)), hi = __ballot(flags_set_high(acc));
+        pending |= (uint64_t)(lo != 0) << g | (uint64_t)(hi != 0) << (g + kHalfShift);
+      } else {
+        const uint64_t vote = __ballot(flags_set(acc));
+        pending |= (uint64_t)(vote != 0) << g;
+      }
 #pragma unroll
       for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
       __syncthreads();
@@ -647,7 +680,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
 // per-wave layout instead (each half of the block an item), which checks its
 // own candidates (the first kernel appended no hits), so the set stays exact.
 constexpr int kResolveWaves = 8;
-// Two blocks per CU (16 waves; the kernel fits 128 VGPRs at QT = 8): a
+// Two blocks per CU (16 waves; the kernel fits 128 VGPRs at its at most 8
+// query tiles per wave): a
 // record's check is a chain of dependent loads, so waves in flight set its
 // rate.  Measured against one block per CU: 19 against 22.5 us at the
 // bench's sparse hits, 578 against 871 us at 271k hits per batch.
@@ -694,6 +728,7 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __
                                                uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
                                                uint64_t capacity, unsigned long long* __restrict__ counter,
                                                uint64_t index_base) {
+  if (pending == 0) return;  // a half of a record with no candidates
   const int lane = threadIdx.x & 63;
   const int rn = lane & 15, kq = lane >> 4;
   const float bound = (float)(128 - 2 * (int)threshold);
@@ -764,6 +799,23 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __
   }
 }
 
+// Query tiles per wave in the recheck: at most eight.  A record of a 16-tile
+// stream is checked as two halves (query tiles 0-7 and 8-15, each with its
+// own pending groups) by two waves: sixteen tiles' operands need more
+// registers than two blocks per CU leave, and a record's check is a chain of
+// dependent steps whose length, not the total work, sets the kernel's time
+// when records are few.
+template <int QT>
+constexpr int kResolveQt = QT > 8 ? 8 : QT;
+
+// The pending groups of part `part` of a record (of kParts parts).
+template <int kParts>
+__device__ __forceinline__ uint32_t part_groups(uint32_t mask, uint32_t part) {
+  static_assert(kParts == 1 || kParts == 2, "whole records or halves of a 16-tile record");
+  if constexpr (kParts == 1) return mask;
+  else return (mask >> (kHalfShift * part)) & ((1u << kHalfShift) - 1);
+}
+
 template <int QT, bool kPlane>
 __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
@@ -776,27 +828,35 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
   const int wave = (int)(threadIdx.x >> 6);
   const uint32_t n_recs = *rec_count;
   if (n_recs > rec_cap) {
+    // the per-wave layout, at most eight query tiles per wave
+    constexpr int kRq = kResolveQt<QT>;
+    const uint32_t rgroups = (n_queries + kRq * 16 - 1) / (kRq * 16);
     constexpr int kHalves = kResolveWaves / kMfmaWaves;
-    const uint64_t items = chunks * groups;
+    const uint64_t items = chunks * rgroups;
     for (uint64_t item = (uint64_t)blockIdx.x * kHalves + wave / kMfmaWaves; item < items;
          item += (uint64_t)gridDim.x * kHalves) {
       uint64_t chunk;
       uint32_t group;
-      item_coords(item, groups, chunk, group);
-      mfma_item<QT, kPlane>(L.lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
-                            counter, index_base, wave % kMfmaWaves);
+      item_coords(item, rgroups, chunk, group);
+      mfma_item<kRq, kPlane>(L.lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
+                             counter, index_base, wave % kMfmaWaves);
     }
+    (void)groups;
     return;
   }
+  // item r: part r % kParts of record r / kParts
+  constexpr int kRq = kResolveQt<QT>, kParts = QT / kRq;
+  const uint32_t n_items = n_recs * kParts;
   const uint32_t stride = gridDim.x * kResolveWaves;
   uint32_t r = blockIdx.x * kResolveWaves + wave;
   // the next record is fetched while this one runs
-  uint4 rec = r < n_recs ? recs[r] : make_uint4(0, 0, 0, 0);
-  for (; r < n_recs; r += stride) {
-    const uint4 next = r + stride < n_recs ? recs[r + stride] : make_uint4(0, 0, 0, 0);
+  uint4 rec = r < n_items ? recs[r / kParts] : make_uint4(0, 0, 0, 0);
+  for (; r < n_items; r += stride) {
+    const uint4 next = r + stride < n_items ? recs[(r + stride) / kParts] : make_uint4(0, 0, 0, 0);
     const uint64_t row0 = ((uint64_t)rec.y << 32 | rec.x) * kSharedRows;
-    resolve_record<QT, kPlane>(L, db, plane, n_rows, row0, rec.z, rec.w, queries, n_queries, threshold, hits,
-                               capacity, counter, index_base);
+    resolve_record<kRq, kPlane>(L, db, plane, n_rows, row0, rec.z + (r % kParts) * (kRq * 16),
+                                part_groups<kParts>(rec.w, r % kParts), queries, n_queries, threshold, hits,
+                                capacity, counter, index_base);
     rec = next;
   }
   resolve_flush(L, hits, capacity, counter);
@@ -858,7 +918,8 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_shards_k
   const uint32_t n_recs = *rec_count;
   if (n_recs > rec_cap) {
     constexpr int kHalves = kResolveWaves / kMfmaWaves;
-    const uint32_t groups = (nq + QT * 16 - 1) / (QT * 16);
+    constexpr int kRq = kResolveQt<QT>;
+    const uint32_t groups = (nq + kRq * 16 - 1) / (kRq * 16);
     const uint64_t items = total_chunks * groups;
     for (uint64_t item = (uint64_t)blockIdx.x * kHalves + wave / kMfmaWaves; item < items;
          item += (uint64_t)gridDim.x * kHalves) {
@@ -866,23 +927,26 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_shards_k
       uint32_t group;
       item_coords(item, groups, chunk, group);
       const ShardView& v = shards[shard_of_chunk(shards, n_shards, chunk)];
-      mfma_item<QT, kPlane>(L.lut, reinterpret_cast<const uint32_t*>(v.rows),
+      mfma_item<kRq, kPlane>(L.lut, reinterpret_cast<const uint32_t*>(v.rows),
                             reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, chunk - v.chunk_begin, group, queries,
                             nq, threshold, hits, capacity, counter, v.index_base, wave % kMfmaWaves);
     }
     return;
   }
+  constexpr int kRq = kResolveQt<QT>, kParts = QT / kRq;
+  const uint32_t n_items = n_recs * kParts;
   const uint32_t stride = gridDim.x * kResolveWaves;
   uint32_t r = blockIdx.x * kResolveWaves + wave;
-  uint4 rec = r < n_recs ? recs[r] : make_uint4(0, 0, 0, 0);
-  for (; r < n_recs; r += stride) {
-    const uint4 next = r + stride < n_recs ? recs[r + stride] : make_uint4(0, 0, 0, 0);
+  uint4 rec = r < n_items ? recs[r / kParts] : make_uint4(0, 0, 0, 0);
+  for (; r < n_items; r += stride) {
+    const uint4 next = r + stride < n_items ? recs[(r + stride) / kParts] : make_uint4(0, 0, 0, 0);
     const uint64_t gsub = (uint64_t)rec.y << 32 | rec.x;
     const ShardView& v = shards[shard_of_chunk(shards, n_shards, gsub / kSubsPerChunk)];
     const uint64_t row0 = (gsub - v.chunk_begin * kSubsPerChunk) * kSharedRows;
-    resolve_record<QT, kPlane>(L, reinterpret_cast<const uint32_t*>(v.rows),
-                               reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, row0, rec.z, rec.w, queries, nq,
-                               threshold, hits, capacity, counter, v.index_base);
+    resolve_record<kRq, kPlane>(L, reinterpret_cast<const uint32_t*>(v.rows),
+                                reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, row0,
+                                rec.z + (r % kParts) * (kRq * 16), part_groups<kParts>(rec.w, r % kParts), queries,
+                                nq, threshold, hits, capacity, counter, v.index_base);
     rec = next;
   }
   resolve_flush(L, hits, capacity, counter);
@@ -1214,7 +1278,10 @@ static int qt_for(int max_qt, uint32_t n_queries) {
   const uint32_t tiles = (n_queries + 15) / 16;
   if (tiles <= 2 || max_qt == 2) return 2;
   if (tiles <= 4 || max_qt == 4) return 4;
-  if (tiles <= 8 || max_qt == 8) return 8;
+  // sixteen tiles per wave only where the shared-rows layout runs (4 query
+  // groups or more): the per-wave layout at sixteen needs more registers
+  // than it has
+  if (tiles < 16 * (uint32_t)kMfmaWaves || max_qt == 8) return 8;
   return 16;
 }
 
@@ -1245,7 +1312,7 @@ static hipError_t launch_match_mfma_qt(int max_qt, const void* db, const void* p
 }
 
 #ifndef YPDQ_DEFAULT_QT
-#define YPDQ_DEFAULT_QT 8
+#define YPDQ_DEFAULT_QT 16
 #endif
 constexpr int kDefaultQt = YPDQ_DEFAULT_QT;
 
@@ -1253,8 +1320,9 @@ hipError_t launch_match_mfma(const void* db, const void* plane, uint64_t n_rows,
                              uint32_t n_queries, uint32_t threshold, MatchHit* hits, uint64_t capacity,
                              unsigned long long* counter, uint64_t index_base, hipStream_t stream,
                              bool reset_counter) {
-  // eight 16-query tiles per wave measured fastest on MI355X (10M rows x
-  // 1024 / 8192 queries); larger tiles cost occupancy
+  // up to sixteen 16-query tiles per wave: sixteen from 64 tiles (the
+  // shared-rows layout) measured 3.5 % faster than eight per call on MI355X
+  // (10M rows x 1024 queries); the per-wave layout stops at eight
   return launch_match_mfma_qt(kDefaultQt, db, plane, n_rows, queries, n_queries, threshold, hits, capacity, counter,
                               index_base, stream, reset_counter, nullptr);
 }
