@@ -58,8 +58,8 @@ struct DropInResult {
 
 // Per-thread state for the drop-in entry points (the reference clones its
 // kernel per call, src/lib.rs:266, so calls are reentrant).  A hash call
-// copies the image into page-locked h_img with memcpy; a copy kernel on the
-// call's stream pulls it into d_img (a DMA-engine copy followed by a kernel
+// copies the image into page-locked h_img with memcpy, in two halves; a copy
+// kernel per half on the call's stream pulls it into d_img (a DMA-engine copy followed by a kernel
 // costs a cross-engine dependency of a few tens of microseconds); the hash
 // kernel writes its results straight into page-locked h_res over the bus: no
 // copy back, one synchronisation per call.  The visit call's 1 KiB PDQF is
@@ -173,11 +173,17 @@ float yume_pdq_hash_smart_kernel(const float* input, float* threshold, uint8_t* 
                                  float* pdqf) {
   DropInState* s = state();
   hipError_t e;
-  memcpy(s->h_img, input, 512 * 512 * sizeof(float));
-  constexpr uint32_t kVec = 512 * 512 / 4;
-  hipLaunchKernelGGL(pull_plane, dim3(kVec / 256), dim3(256), 0, s->stream, reinterpret_cast<pull_f4*>(s->d_img),
-                     reinterpret_cast<const pull_f4*>(s->dv_img), kVec);
-  if ((e = hipGetLastError()) != hipSuccess) die("copy kernel launch", e);
+  // in two halves: the device pulls the first while the host copies the
+  // second (77.7 against 95.6 us per call in one piece, same box)
+  constexpr int kHalves = 2;
+  constexpr uint32_t kVec = 512 * 512 / 4 / kHalves;  // 16-byte elements per half
+  for (int h = 0; h < kHalves; h++) {
+    memcpy(s->h_img + h * kVec * 4, input + h * kVec * 4, kVec * sizeof(pull_f4));
+    hipLaunchKernelGGL(pull_plane, dim3(64), dim3(256), 0, s->stream,
+                       reinterpret_cast<pull_f4*>(s->d_img) + h * kVec,
+                       reinterpret_cast<const pull_f4*>(s->dv_img) + h * kVec, kVec);
+    if ((e = hipGetLastError()) != hipSuccess) die("copy kernel launch", e);
+  }
   HashOutputs o = outputs_of(s->dv_res, true);
   o.dihedral_pdqf = nullptr;
   if ((e = ypdq::launch_hash(s->d_img, ypdq::kInputF32, 512 * 512 * sizeof(float), 1, ypdq::kDihedralAvx2, o,
