@@ -151,6 +151,26 @@ def test_block_single_and_auto(gpu, dev, imgs):
     assert_same(a, run(gpu, x, gpu.LAYOUT_AUTO), "auto")
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_wave_layout_sixteen_wave_blocks(gpu, dev, imgs, mode):
+    """A u8 batch of 16 images per CU or more runs the wave-per-image kernel
+    in blocks of sixteen waves (one block per CU, four waves per SIMD): 16 x
+    CUs + 7 images, the last block partly empty, equal the eight-wave block
+    kernel image for image, buf1 included."""
+    import torch
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    n = 16 * cus + 7
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234 + mode)
+    x = torch.randint(0, 256, (n, 512, 512), dtype=torch.uint8, device=dev, generator=g)
+    x[: imgs.shape[0]] = torch.from_numpy(imgs).to(dev)
+    x[-imgs.shape[0]:] = torch.from_numpy(imgs[::-1].copy()).to(dev)
+    say(f"[hash] sixteen-wave blocks: n={n} cus={cus} mode={mode}")
+    a = run(gpu, x, gpu.LAYOUT_WAVE, mode=mode)
+    b = run(gpu, x, gpu.LAYOUT_BLOCK8, mode=mode)
+    assert_same(a, b, f"sixteen-wave blocks n={n} mode={mode}")
+
+
 def dct64_matrix():
     i = np.arange(1, 17, dtype=np.float64)[:, None]
     j = np.arange(127, dtype=np.float64)[None, :]

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pdq_hash_common.h"
 #include "pdq_hash_tail.h"
 
@@ -88,11 +90,10 @@ __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lan
   cur = nxt;
 }
 
-template <typename T>
+template <typename T, int D>
 __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int lane, float (&acc)[16][2][2],
                                              float* __restrict__ buf1, const Luma& lc) {
   using Raw = typename RawRow<T>::type;
-  constexpr int D = Prefetch<T>::D;
   static_assert(4 % D == 0, "ring depth must divide the unroll factor");
 #pragma unroll
   for (int k = 0; k < 16; k++)
@@ -244,16 +245,15 @@ __device__ __forceinline__ bool plane_needs_literal(const Rgba8* __restrict__, i
   return false;
 }
 
-template <typename T>
+template <typename T, int D, typename Lds>
 __device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t plane_stride, uint32_t img, int lane,
-                                         int wv, uint32_t mode, const HashOutputs& out, BlockLds& L,
-                                         const Luma& lc) {
+                                         int wv, uint32_t mode, const HashOutputs& out, Lds& L, const Luma& lc) {
   const T* plane = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(planes) + (size_t)img * plane_stride);
   float* buf1 = out.buf1 ? out.buf1 + (size_t)img * 128 * 128 : nullptr;
   WaveLds& W = L.w[wv];
   {
     float acc[16][2][2];
-    stream_image(plane, lane, acc, buf1, lc);
+    stream_image<T, D>(plane, lane, acc, buf1, lc);
     const bool literal = plane_needs_literal(plane, lane, acc);
     write_stage1(acc, W, lane);
     if (literal) stream_image_literal(plane, lane, W, buf1, lc);
@@ -269,17 +269,20 @@ __device__ __forceinline__ void hash_one(const T* __restrict__ planes, uint64_t 
   dihedral_tail(W, lane, mode, out, img, nullptr);
 }
 
-// One wave per image; the grid covers n_images.  With d_n set, the image
-// count is read from device memory (capped at n_images) and the waves past it
-// return at once, so one captured launch sized for the largest batch serves
-// batches of any size.
-template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) pdq_hash_kernel(const T* __restrict__ planes, uint64_t plane_stride,
-                                                        uint32_t n_images, const uint32_t* __restrict__ d_n,
-                                                        uint32_t mode, HashOutputs out, Luma lc) {
+// One wave per image, kW images per block; the grid covers n_images.  With
+// d_n set, the image count is read from device memory (capped at n_images)
+// and the waves past it return at once, so one captured launch sized for the
+// largest batch serves batches of any size.  Blocks of 4 waves run 3 per CU
+// (46.7 KB of LDS and 168 registers each); blocks of 16 (161.5 KB) run one
+// per CU, four waves per SIMD within 128 registers, which u8 rows reach with
+// one prefetch step in flight (D = 1).
+template <typename T, int kW, int D>
+__global__ void __launch_bounds__(64 * kW) __attribute__((amdgpu_waves_per_eu(kW >= 16 ? 4 : 3))) pdq_hash_kernel(
+    const T* __restrict__ planes, uint64_t plane_stride, uint32_t n_images, const uint32_t* __restrict__ d_n,
+    uint32_t mode, HashOutputs out, Luma lc) {
   const uint32_t n = d_n ? min(*d_n, n_images) : n_images;
-  if (blockIdx.x * kWavesPerBlock >= n) return;
-  __shared__ BlockLds L;
+  if (blockIdx.x * kW >= n) return;
+  __shared__ BlockLdsOf<kW> L;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   // padded DCT rows for stage 2: d2[j][m] = D_flat[j*127 + m] (m = 127 reads
@@ -289,9 +292,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) p
     L.d2[j * kTS + m] = kDct[j * 127 + m];
   }
   __syncthreads();
-  const uint32_t img = blockIdx.x * kWavesPerBlock + wv;
+  const uint32_t img = blockIdx.x * kW + wv;
   if (img >= n) return;
-  hash_one(planes, plane_stride, img, lane, wv, mode, out, L, lc);
+  hash_one<T, D>(planes, plane_stride, img, lane, wv, mode, out, L, lc);
 }
 
 // Dihedrals of given PDQFs (the visit entry point): pdqf [n][256], thr_in [n].
@@ -322,12 +325,33 @@ hipError_t launch_dihedrals(const float* pdqf, const float* thr_in, uint32_t n, 
 
 
 
+template <typename T, int kW, int D>
+static void launch_kind_w(const void* planes, uint64_t plane_stride, uint32_t n_max, const uint32_t* d_n,
+                          uint32_t mode, const HashOutputs& out, const Luma& lc, hipStream_t stream) {
+  const dim3 grid((n_max + kW - 1) / kW);
+  hipLaunchKernelGGL((pdq_hash_kernel<T, kW, D>), grid, dim3(64 * kW), 0, stream, static_cast<const T*>(planes),
+                     plane_stride, n_max, d_n, mode, out, lc);
+}
+
+// u8 batches of at least 16 images per CU take the 16-wave blocks.
+static bool sixteen_wave_blocks(uint32_t n_max) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess)
+    return false;
+  return cus > 0 && n_max >= 16u * (uint32_t)cus;
+}
+
 template <typename T>
 static void launch_kind(const void* planes, uint64_t plane_stride, uint32_t n_max, const uint32_t* d_n, uint32_t mode,
                         const HashOutputs& out, const Luma& lc, hipStream_t stream) {
-  const dim3 grid((n_max + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL(pdq_hash_kernel<T>, grid, dim3(64 * kWavesPerBlock), 0, stream, static_cast<const T*>(planes),
-                     plane_stride, n_max, d_n, mode, out, lc);
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    if (sixteen_wave_blocks(n_max)) {
+      launch_kind_w<T, 16, 1>(planes, plane_stride, n_max, d_n, mode, out, lc, stream);
+      return;
+    }
+  }
+  launch_kind_w<T, kWavesPerBlock, Prefetch<T>::D>(planes, plane_stride, n_max, d_n, mode, out, lc, stream);
 }
 
 static hipError_t launch_any(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,

@@ -301,10 +301,12 @@ struct WaveLds {
 
 constexpr int kWavesPerBlock = 4;
 
-struct BlockLds {
+template <int kW>
+struct BlockLdsOf {
   float d2[16 * kTS];  // DCT rows padded to 128 (slot 127 = the reference's over-read), stride kTS
-  WaveLds w[kWavesPerBlock];
+  WaveLds w[kW];
 };
+using BlockLds = BlockLdsOf<kWavesPerBlock>;
 
 // 8-lane group reductions with the AVX2 tree shapes.  Lanes of a group are
 // g*8 .. g*8+7.  The operand moves are DPP lane permutations within rows of
