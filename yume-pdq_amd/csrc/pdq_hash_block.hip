@@ -377,13 +377,14 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const T* plane = reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(planes) + (size_t)img * plane_stride);
   float* c = L.u.c;
-  if (kPrec == kDctExact || std::is_same<T, float>::value)
-    for (int e = t; e < 127 * 16; e += 64 * kWaves) L.dct_t[e] = kDctT.v[e];
 
-  // A. compression
+  // A. compression; the DCT table for stage 1 is copied into LDS after it,
+  // so the first plane rows are requested at the kernel's start
   const int i0 = wv * kBand, i1 = min(i0 + kBand, 127);
   bool bad = compress_band(plane, lane, i0, i1, c, lc);
   if (wv == kWaves - 1) *reinterpret_cast<float2*>(&c[127 * 128 + 2 * lane]) = make_float2(0.0f, 0.0f);
+  if (kPrec == kDctExact || std::is_same<T, float>::value)
+    for (int e = t; e < 127 * 16; e += 64 * kWaves) L.dct_t[e] = kDctT.v[e];
   if constexpr (std::is_same<T, float>::value) bad = bad || edges_nonfinite(plane, t, 64 * kWaves);
   const bool literal = __syncthreads_or(bad ? 1 : 0) != 0;
   if constexpr (std::is_same<T, float>::value) {
