@@ -120,13 +120,18 @@ def main():
     d_hits = torch.empty((cap, 16), dtype=torch.uint8, device=dev)
     d_count = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
+    # the second in-flight step's stream (default priority, as every stream here)
+    streams = (stream, torch.cuda.Stream(device=dev))
 
     hm_cache = {}
 
     def hash_match_for(planes, B, plane, slot=0):
+        """The step's one-call object for (planes, B, plane); slot 1 runs on the
+        second stream."""
         key = (planes.data_ptr(), planes.dtype, B, plane is not None, slot)
         if key not in hm_cache:
-            hm_cache[key] = yume_pdq.HashMatch(planes[:B], d_db, thr, plane=plane, capacity=1 << 20, stream=stream)
+            hm_cache[key] = yume_pdq.HashMatch(planes[:B], d_db, thr, plane=plane, capacity=1 << 20,
+                                               stream=streams[slot])
         return hm_cache[key]
 
     def step(planes, B, ev=None, plane=d_plane):
@@ -165,12 +170,13 @@ def main():
         t1 = time.perf_counter()
         t_gpu = [e[0].elapsed_time(e[1]) for e in evs]
         sync_s = t1 - t0
-        # The same K steps with two in flight: step k + 1 is enqueued before
-        # the host waits for step k's hit count, so the host's wait, the
-        # count read and the next call's enqueue overlap the GPU work (each
-        # step has its own outputs; the GPU still runs the steps one after
-        # another).  This is the throughput; the latency pass below runs
-        # steps one at a time.
+        # The same K steps with two in flight, on two streams: step k + 1 is
+        # enqueued before the host waits for step k's hit count, so the
+        # host's wait, the count read and the next call's enqueue overlap the
+        # GPU work, and the GPU may start step k + 1's kernels while step k's
+        # last blocks finish (each step has its own outputs and record area).
+        # This is the throughput; the latency pass below runs steps one at a
+        # time.
         hms = [hash_match_for(planes, B, plane, slot=k) for k in (0, 1)]
         done = [torch.cuda.Event() for _ in range(2)]
         counts = []
@@ -179,7 +185,7 @@ def main():
             t0 = time.perf_counter()
             for k in range(args.steps):
                 hms[k & 1].run(sync=False)
-                done[k & 1].record(stream)
+                done[k & 1].record(streams[k & 1])
                 if k >= 1:
                     done[(k - 1) & 1].synchronize()
                     counts.append(int(hms[(k - 1) & 1].h_count[0]))
