@@ -104,6 +104,36 @@ def test_batcher_uneven_batches(gpu, dev, db_parts, shard_set):
         b.close()
 
 
+def test_batcher_large_u8_context(gpu, dev, db_parts, shard_set):
+    """A batch context of at least 16 u8 images per CU hashes in 16-wave
+    blocks with the batch size read from device memory: a full batch and a
+    small one through the same graphs equal the eight-wave block kernel's
+    hashes, and their hit counts equal match on the concatenated database."""
+    import torch
+    parts, _ = db_parts
+    full = torch.from_numpy(np.concatenate(parts)).to(dev)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    cap = 16 * cus + 8
+    g = torch.Generator(device=dev)
+    g.manual_seed(77)
+    imgs = torch.randint(0, 256, (cap, 512, 512), dtype=torch.uint8, device=dev, generator=g)
+    imgs[:64] = torch.from_numpy(synthetic.images_u8(64)).to(dev)
+    b = gpu.Batcher(shard_set, cap, threshold=31, hit_capacity=1 << 20)
+    try:
+        for n in (cap - 5, 5, cap):
+            say(f"[batcher] large u8 context: n={n} of {cap}")
+            planes = imgs[:n]
+            h, qv, t, total, _ = b.run(planes)
+            eh, eq, et = gpu.hash_batch(planes, layout=gpu.LAYOUT_BLOCK8)
+            assert torch.equal(h, eh)
+            assert torch.equal(qv.view(torch.int32), eq.view(torch.int32))
+            assert torch.equal(t.view(torch.int32), et.view(torch.int32))
+            en, _ = gpu.match(full, eh.reshape(-1, 32), 31, capacity=1 << 20)
+            assert total == en
+    finally:
+        b.close()
+
+
 def test_batcher_planted_hits_found(gpu, dev, db_parts, shard_set):
     """Queries that are the hashes of real images: plant near-duplicates of
     them into a shard through in-place replacement and find every one."""
