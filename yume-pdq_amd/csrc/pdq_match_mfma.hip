@@ -278,6 +278,77 @@ __device__ __forceinline__ bool flags_set_high(const uint32_t (&acc)[kFlagChains
   return ((acc[2] | acc[3]) & kFlagMask) != 0;
 }
 
+// Three products per accumulator (sixteen query tiles, thresholds 28..63).
+// The products of one row tile with query tiles a, b, c add dot_a (scale
+// 2^0), 2^7 dot_b and 2^15 dot_c to
+//     C0 = 2^23 + K_a + 2^8 K_b + 2^16 K_c,
+//     K_a = 128 - bound,  K_b = 129 - bound/2,  K_c = 64 - bound/2,
+// so, dot being even, the result is 2^23 + f_a + 2^8 f_b + 2^16 f_c with
+// f_a = dot_a + K_a, f_b = dot_b/2 + K_b, f_c = dot_c/2 + K_c.  Every
+// partial sum, in any order, is an integer in (2^22, 2^24): exact.
+//   bit 7:  f_a >= 128, i.e. dot_a >= bound (f_a <= 254: no carry);
+//   bit 15: f_b >= 128, i.e. dot_b >= bound - 2 (f_b in [1, 192]);
+//   bit 22: f_c >= 64, i.e. dot_c >= bound (f_c <= 127).
+// f_a or f_c below zero (dot < bound - 128: a prefix distance above 64 + t,
+// about 6e-9 of random pairs at t = 31) borrows: a borrow from f_a shows
+// in bit 7 (f_a + 256 >= 128) and lowers f_b by one, which the slack of
+// K_b (a flag from dot_b >= bound - 2) absorbs; a borrow from f_c takes the
+// result below 2^23, which changes the exponent's lowest bit, bit 23.  So a
+// pair at the bound always sets one of the four bits, and the other causes
+// are rare false candidates that the recheck discards.  Checked over every
+// (dot_a, dot_b, dot_c) and threshold by a host-side model of the fields.
+// Per row tile the sixteen query tiles form accumulators of (0,1,2),
+// (3,4,5), (6,7), (8,9,10), (11,12,13), (14,15): twelve folds instead of
+// sixteen; accumulators 0-2 hold the first half of the query tiles.
+constexpr uint32_t kFlagMaskTri = (1u << 7) | (1u << 15) | (1u << 22) | (1u << 23);
+constexpr uint32_t kTriMinThreshold = 28;  // below it, distant pairs borrow too often
+constexpr int kScale128 = 127 + 7;         // 2^7
+constexpr int kScale32k = 127 + 15;        // 2^15
+
+__device__ __forceinline__ v4f flags_c0_tri(float bound) {
+  const int b = (int)bound;
+  const int c = 8388608 + (128 - b) + 256 * (129 - b / 2) + 65536 * (64 - b / 2);  // below 2^24: exact
+  const float f = (float)c;
+  return v4f{f, f, f, f};
+}
+
+// Accumulator m (0..5) of a row tile: its first query tile and its size.
+__device__ constexpr int tri_first(int m) { return m < 3 ? 3 * m : 8 + 3 * (m - 3); }
+__device__ constexpr int tri_size(int m) { return (m == 2 || m == 5) ? 2 : 3; }
+// Flag chain of accumulator P: two per half of the query tiles.
+__device__ constexpr int tri_chain(int p) { return 2 * ((p % 6) >= 3 ? 1 : 0) + (p & 1); }
+
+// Accumulator P (row tile P / 6, accumulator P % 6): its first product issues
+// at step P into result register P % 4, its second at step P + 1 and its
+// third at step P + 2, and its bits are folded at step P + 4, before the
+// register takes accumulator P + 4.
+template <int kTiles, int S>
+__device__ __forceinline__ void group_flags_tri_step(const Expanded (&qa)[16], const Expanded (&rb)[kTiles],
+                                                     const v4f& c0, uint32_t (&acc)[kFlagChains], v4f& r0, v4f& r1,
+                                                     v4f& r2, v4f& r3) {
+  constexpr int kN = 6 * kTiles;  // accumulators
+  v4f& cur = (S % 4 == 0) ? r0 : (S % 4 == 1) ? r1 : (S % 4 == 2) ? r2 : r3;
+  v4f& p1 = (S % 4 == 1) ? r0 : (S % 4 == 2) ? r1 : (S % 4 == 3) ? r2 : r3;
+  v4f& p2 = (S % 4 == 2) ? r0 : (S % 4 == 3) ? r1 : (S % 4 == 0) ? r2 : r3;
+  if constexpr (S >= 4) fold_flags(cur, acc[tri_chain(S - 4)]);
+  if constexpr (S < kN) cur = mfma_acc<kScale1>(qa[tri_first(S % 6)], rb[S / 6], c0);
+  if constexpr (S >= 1 && S - 1 < kN)
+    p1 = mfma_acc<kScale128>(qa[tri_first((S - 1) % 6) + 1], rb[(S - 1) / 6], p1);
+  if constexpr (S >= 2 && S - 2 < kN) {
+    if constexpr (tri_size((S - 2) % 6) == 3)
+      p2 = mfma_acc<kScale32k>(qa[tri_first((S - 2) % 6) + 2], rb[(S - 2) / 6], p2);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (S + 1 < kN + 4) group_flags_tri_step<kTiles, S + 1>(qa, rb, c0, acc, r0, r1, r2, r3);
+}
+
+template <int kTiles>
+__device__ __forceinline__ void group_flags_tri(const Expanded (&qa)[16], const Expanded (&rb)[kTiles], const v4f& c0,
+                                                uint32_t (&acc)[kFlagChains]) {
+  v4f r0, r1, r2, r3;
+  group_flags_tri_step<kTiles, 0>(qa, rb, c0, acc, r0, r1, r2, r3);
+}
+
 // The rare path: recompute one tile's products and re-check every pair that
 // clears the bound with the exact 256-bit distance.
 template <int QT>
@@ -467,7 +538,7 @@ struct SharedRowsLds {
 // exist, every sub-chunk but a database's last) has a loop with fixed bounds
 // and no branch: a branch in the loop makes the compiler wait for every load
 // in flight at each group.
-template <int QT, bool kFull, bool kPlane>
+template <int QT, bool kFull, bool kPlane, bool kTri>
 __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const uint32_t* __restrict__ lut,
                                               SharedRowsLds& S, const RowSource<kPlane>& src,
                                               const uint32_t* __restrict__ db, uint64_t row0, uint64_t n_rows,
@@ -524,7 +595,8 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
   uint32_t acc[kFlagChains];
 #pragma unroll
   for (int m = 0; m < kFlagChains; m++) acc[m] = 0u;
-  const v4f c0 = flags_c0(bound);
+  static_assert(!kTri || QT == 16, "three products per accumulator at sixteen query tiles");
+  const v4f c0 = kTri ? flags_c0_tri(bound) : flags_c0(bound);
   uint64_t pending = 0;  // groups of tiles whose products reached the bound
 #pragma unroll 1
   for (uint32_t g0 = 0; g0 < ngroups; g0 += kRing) {
@@ -556,7 +628,10 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
         rb[j].d[2] = vt[j].z;
         rb[j].d[3] = vt[j].w;
       }
-      group_flags<QT, kGroupTiles>(qa, rb, c0, acc);
+      if constexpr (kTri)
+        group_flags_tri<kGroupTiles>(qa, rb, c0, acc);
+      else
+        group_flags<QT, kGroupTiles>(qa, rb, c0, acc);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int st = 0; st < kStagePerWave; st++) {
@@ -566,7 +641,11 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
       // the vote is a wave-uniform mask, so the pending bits stay in scalar
       // registers; at QT = 16 one vote per half of the query tiles, the
       // second half's bits kHalfShift above the first's
-      if constexpr (QT == 16) {
+      if constexpr (kTri) {
+        const uint64_t lo = __ballot(((acc[0] | acc[1]) & kFlagMaskTri) != 0);
+        const uint64_t hi = __ballot(((acc[2] | acc[3]) & kFlagMaskTri) != 0);
+        pending |= (uint64_t)(lo != 0) << g | (uint64_t)(hi != 0) << (g + kHalfShift);
+      } else if constexpr (QT == 16) {
         const uint64_t lo = __ballot(flags_set_low(acc)), hi = __ballot(flags_set_high(acc));
         pending |= (uint64_t)(lo != 0) << g | (uint64_t)(hi != 0) << (g + kHalfShift);
       } else {
@@ -589,7 +668,7 @@ __device__ __forceinline__ void stream_shared(const Expanded (&qa)[QT], const ui
   }
 }
 
-template <int QT, bool kPlane>
+template <int QT, bool kPlane, bool kTri>
 __device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lut, SharedRowsLds& S,
                                                  const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane,
                                                  uint64_t n_rows, uint64_t sub, uint32_t sgroup,
@@ -622,11 +701,11 @@ __device__ __forceinline__ void mfma_item_shared(const uint32_t* __restrict__ lu
 
   const RowSource<kPlane> src(db, plane, row0, lane);
   if (row0 + kSharedRows <= n_rows)
-    stream_shared<QT, true, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
-                                    rec_cap, rec_sub_base);
+    stream_shared<QT, true, kPlane, kTri>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs,
+                                          rec_count, rec_cap, rec_sub_base);
   else
-    stream_shared<QT, false, kPlane>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs, rec_count,
-                                     rec_cap, rec_sub_base);
+    stream_shared<QT, false, kPlane, kTri>(qa, lut, S, src, db, row0, n_rows, qbase, wave, lane, bound, recs,
+                                           rec_count, rec_cap, rec_sub_base);
 }
 
 __device__ __forceinline__ void init_lut(uint32_t* lut) {
@@ -649,7 +728,7 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
 // per-wave layout, which checks its candidates itself), or, with kShared
 // (4+ query groups), 1024-row sub-chunks x groups of kMfmaWaves query groups,
 // whose candidates become records for pdq_match_resolve_kernel.
-template <int QT, bool kPlane, bool kShared>
+template <int QT, bool kPlane, bool kShared, bool kTri = false>
 __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
@@ -662,8 +741,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
   item_coords(blockIdx.x, groups, chunk, group);
   if constexpr (kShared) {
     __shared__ SharedRowsLds S;
-    mfma_item_shared<QT, kPlane>(lut, S, db, plane, n_rows, chunk, group, queries, n_queries, threshold, recs,
-                                 rec_count, rec_cap, 0);
+    mfma_item_shared<QT, kPlane, kTri>(lut, S, db, plane, n_rows, chunk, group, queries, n_queries, threshold,
+                                       recs, rec_count, rec_cap, 0);
   } else {
     mfma_item<QT, kPlane>(lut, db, plane, n_rows, chunk, group, queries, n_queries, threshold, hits, capacity,
                           counter, index_base, (int)(threadIdx.x >> 6));
@@ -876,7 +955,7 @@ __device__ __forceinline__ uint32_t shard_of_chunk(const ShardView* __restrict__
 // kSubsPerChunk.  The query count is read from device memory (`d_nq`, capped
 // at nq_cap), so one captured launch serves batches of any size.  Records
 // carry the global sub-chunk number.
-template <int QT, bool kPlane>
+template <int QT, bool kPlane, bool kTri = false>
 __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_shards_shared_kernel(
     const ShardView* __restrict__ shards, uint32_t n_shards, uint64_t total_chunks,
     const uint32_t* __restrict__ queries, const uint32_t* __restrict__ d_nq, uint32_t nq_cap, uint32_t threshold,
@@ -895,9 +974,9 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_p
     item_coords(item, sgroups, gsub, sgroup);
     const ShardView& v = shards[shard_of_chunk(shards, n_shards, gsub / kSubsPerChunk)];
     const uint64_t sub_base = v.chunk_begin * kSubsPerChunk;
-    mfma_item_shared<QT, kPlane>(lut, S, reinterpret_cast<const uint32_t*>(v.rows),
-                                 reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, gsub - sub_base, sgroup, queries,
-                                 nq, threshold, recs, rec_count, rec_cap, sub_base);
+    mfma_item_shared<QT, kPlane, kTri>(lut, S, reinterpret_cast<const uint32_t*>(v.rows),
+                                       reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, gsub - sub_base, sgroup,
+                                       queries, nq, threshold, recs, rec_count, rec_cap, sub_base);
   }
 }
 
@@ -1127,10 +1206,20 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       const uint64_t blocks = subs8 * sgroups;
       if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
       hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, ra.count, reset ? counter : nullptr);
-      if (plane)
+      // three products per accumulator where they apply
+      const bool tri = QT == 16 && threshold >= kTriMinThreshold;
+      if (plane && tri)
+        hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true, QT == 16>), dim3((unsigned)blocks), block, 0,
+                           stream, d, p, n_rows, q, n_queries, threshold, hits, capacity, counter, index_base,
+                           sgroups, ra.recs, ra.count, cap);
+      else if (plane)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
                            ra.count, cap);
+      else if (tri)
+        hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true, QT == 16>), dim3((unsigned)blocks), block, 0,
+                           stream, d, p, n_rows, q, n_queries, threshold, hits, capacity, counter, index_base,
+                           sgroups, ra.recs, ra.count, cap);
       else
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
@@ -1238,14 +1327,23 @@ static hipError_t launch_shards_shared_qt(const ShardView* shards, uint32_t n_sh
   const dim3 block(64 * kMfmaWaves), rblock(64 * kResolveWaves);
   const uint32_t cap = record_cap();
   hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, count, nullptr);
+  const bool tri = QT == 16 && threshold >= kTriMinThreshold;  // three products per accumulator
   if (plane) {
-    hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, true>), dim3(grid), block, 0, stream, shards, n_shards,
-                       total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    if (tri)
+      hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, true, QT == 16>), dim3(grid), block, 0, stream,
+                         shards, n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    else
+      hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, true>), dim3(grid), block, 0, stream, shards,
+                         n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
     hipLaunchKernelGGL((pdq_match_resolve_shards_kernel<QT, true>), dim3(rgrid), rblock, 0, stream, shards, n_shards,
                        total_chunks, q, d_nq, nq_cap, threshold, hits, capacity, counter, recs, count, cap);
   } else {
-    hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, false>), dim3(grid), block, 0, stream, shards,
-                       n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    if (tri)
+      hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, false, QT == 16>), dim3(grid), block, 0, stream,
+                         shards, n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
+    else
+      hipLaunchKernelGGL((pdq_match_mfma_shards_shared_kernel<QT, false>), dim3(grid), block, 0, stream, shards,
+                         n_shards, total_chunks, q, d_nq, nq_cap, threshold, recs, count, cap);
     hipLaunchKernelGGL((pdq_match_resolve_shards_kernel<QT, false>), dim3(rgrid), rblock, 0, stream, shards, n_shards,
                        total_chunks, q, d_nq, nq_cap, threshold, hits, capacity, counter, recs, count, cap);
   }
