@@ -128,6 +128,44 @@ def test_mfma_dense_candidates(gpu, oracle, dev, plane):
     assert np.array_equal(sort_triples(as_triples(hits)), exp)
 
 
+@pytest.mark.parametrize("threshold", [0, 20, 27, 28, 29, 31, 45, 62, 63])
+@pytest.mark.parametrize("plane", [False, True])
+def test_mfma_sixteen_tiles_thresholds(gpu, oracle, dev, threshold, plane):
+    """2048 queries (sixteen query tiles per wave, rows shared) at thresholds
+    on both sides of where three products share an accumulator (28), with
+    pairs built against its flag fields: row p sits exactly at the bound
+    from query 16 + p (query tile 1, the second product of an accumulator),
+    while queries p and 32 + p (tiles 0 and 2, the same accumulator and
+    lane) are near-complements of its prefix, whose fields borrow; the same
+    again for tiles 8-10.  Plus 300 plants at distances up to 70 among 4000
+    random rows.  Every set equals the oracle's."""
+    import torch
+    rng = np.random.default_rng(500 + threshold)
+    q = synthetic.random_db(2048, seed=501)
+    rows = []
+    for base in (0, 128):  # query tiles 0-2 and 8-10
+        for p in range(16):
+            target = q[base + 16 + p].copy()
+            row = target.copy()
+            for b in rng.choice(128, size=threshold, replace=False):  # prefix bits only: d256 = threshold
+                row[b // 8] ^= np.uint8(1 << (b % 8))
+            rows.append(row)
+            for off in (0, 32):  # prefix distance 128 - k from the row: far past 64 + t
+                comp = row.copy()
+                comp[:16] ^= np.uint8(0xFF)
+                for b in rng.choice(128, size=int(rng.integers(0, 4)), replace=False):
+                    comp[b // 8] ^= np.uint8(1 << (b % 8))
+                q[base + off + p] = comp
+    db = np.concatenate([synthetic.random_db(4000, seed=502), np.stack(rows)])
+    synthetic.plant(db[:4000], q, 300, seed=503, max_dist=70)
+    say(f"[match] sixteen tiles threshold={threshold} plane={plane}")
+    exp_n, exp = oracle.match_all(db, q, threshold, cap=1 << 20)
+    total, hits = run_path(gpu, torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev), threshold,
+                           "mfma_plane" if plane else "mfma", capacity=1 << 20)
+    assert total == exp_n and exp_n >= 32  # the 32 rows built at the bound
+    assert np.array_equal(sort_triples(as_triples(hits)), exp)
+
+
 def near(base, n, max_flips, seed):
     """n copies of the 32-byte hash `base`, each with up to max_flips bits flipped."""
     rng = np.random.default_rng(seed)

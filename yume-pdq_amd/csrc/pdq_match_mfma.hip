@@ -295,8 +295,8 @@ __device__ __forceinline__ bool flags_set_high(const uint32_t (&acc)[kFlagChains
 // K_b (a flag from dot_b >= bound - 2) absorbs; a borrow from f_c takes the
 // result below 2^23, which changes the exponent's lowest bit, bit 23.  So a
 // pair at the bound always sets one of the four bits, and the other causes
-// are rare false candidates that the recheck discards.  Checked over every
-// (dot_a, dot_b, dot_c) and threshold by a host-side model of the fields.
+// are rare false candidates that the recheck discards.  tests/test_flag_fields.py
+// checks this over every (dot_a, dot_b, dot_c) and threshold.
 // Per row tile the sixteen query tiles form accumulators of (0,1,2),
 // (3,4,5), (6,7), (8,9,10), (11,12,13), (14,15): twelve folds instead of
 // sixteen; accumulators 0-2 hold the first half of the query tiles.
