@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--plants", type=int, default=50_000)
     ap.add_argument("--batches", default="1,16,128,1024")
     ap.add_argument("--headline-batch", type=int, default=128)
+    ap.add_argument("--in-flight", type=int, default=2,
+                    help="steps in flight in the throughput loop, each on its own stream")
     ap.add_argument("--latency-steps", type=int, default=2000,
                     help="steps of the separate pass the headline p50/p99 come from")
     ap.add_argument("--ramp-ms", type=float, default=400.0,
@@ -120,14 +122,15 @@ def main():
     d_hits = torch.empty((cap, 16), dtype=torch.uint8, device=dev)
     d_count = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
-    # the second in-flight step's stream (default priority, as every stream here)
-    streams = (stream, torch.cuda.Stream(device=dev))
+    # one stream per in-flight step (default priority, as every stream here)
+    in_flight = max(1, args.in_flight)
+    streams = [stream] + [torch.cuda.Stream(device=dev) for _ in range(in_flight - 1)]
 
     hm_cache = {}
 
     def hash_match_for(planes, B, plane, slot=0):
-        """The step's one-call object for (planes, B, plane); slot 1 runs on the
-        second stream."""
+        """The step's one-call object for (planes, B, plane); slot k runs on
+        stream k."""
         key = (planes.data_ptr(), planes.dtype, B, plane is not None, slot)
         if key not in hm_cache:
             hm_cache[key] = yume_pdq.HashMatch(planes[:B], d_db, thr, plane=plane, capacity=1 << 20,
@@ -170,27 +173,30 @@ def main():
         t1 = time.perf_counter()
         t_gpu = [e[0].elapsed_time(e[1]) for e in evs]
         sync_s = t1 - t0
-        # The same K steps with two in flight, on two streams: step k + 1 is
+        # The same K steps with F in flight (default two; three and four
+        # measured slower), each slot on its own stream: step k + 1 is
         # enqueued before the host waits for step k's hit count, so the
         # host's wait, the count read and the next call's enqueue overlap the
         # GPU work, and the GPU may start step k + 1's kernels while step k's
         # last blocks finish (each step has its own outputs and record area).
         # This is the throughput; the latency pass below runs steps one at a
         # time.
-        hms = [hash_match_for(planes, B, plane, slot=k) for k in (0, 1)]
-        done = [torch.cuda.Event() for _ in range(2)]
+        F = in_flight
+        hms = [hash_match_for(planes, B, plane, slot=k) for k in range(F)]
+        done = [torch.cuda.Event() for _ in range(F)]
         counts = []
         for rep in range(2):  # an untimed round, then the timed one
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for k in range(args.steps):
-                hms[k & 1].run(sync=False)
-                done[k & 1].record(streams[k & 1])
-                if k >= 1:
-                    done[(k - 1) & 1].synchronize()
-                    counts.append(int(hms[(k - 1) & 1].h_count[0]))
-            done[(args.steps - 1) & 1].synchronize()
-            counts.append(int(hms[(args.steps - 1) & 1].h_count[0]))
+                if k >= F:  # step k - F, on this slot, is done: its count is read first
+                    done[k % F].synchronize()
+                    counts.append(int(hms[k % F].h_count[0]))
+                hms[k % F].run(sync=False)
+                done[k % F].record(streams[k % F])
+            for k in range(max(0, args.steps - F), args.steps):
+                done[k % F].synchronize()
+                counts.append(int(hms[k % F].h_count[0]))
             torch.cuda.synchronize()
             t1 = time.perf_counter()
         assert all(c == hits for c in counts), "pipelined steps found different hit counts"
