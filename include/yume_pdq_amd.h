@@ -128,14 +128,16 @@ enum {
   YUME_PDQ_AMD_DCT_BF16X3 = 3,
   YUME_PDQ_AMD_DCT_F16X3 = 4
 };
-/* Kernel layout: one wavefront per image, or 4 / 8 wavefronts sharing an
- * image (lower latency per image, fills the GPU with fewer images).  AUTO
- * picks by batch size.  The exact results do not depend on the layout. */
+/* Kernel layout: one wavefront per image, or 4 / 8 / 16 wavefronts sharing
+ * an image (lower latency per image, fills the GPU with fewer images; 16:
+ * exact arithmetic only).  AUTO picks by batch size.  The exact results do
+ * not depend on the layout. */
 enum {
   YUME_PDQ_AMD_LAYOUT_AUTO = 0,
   YUME_PDQ_AMD_LAYOUT_WAVE = 1,
   YUME_PDQ_AMD_LAYOUT_BLOCK4 = 2,
-  YUME_PDQ_AMD_LAYOUT_BLOCK8 = 3
+  YUME_PDQ_AMD_LAYOUT_BLOCK8 = 3,
+  YUME_PDQ_AMD_LAYOUT_BLOCK16 = 4
 };
 typedef struct YumePdqAmdHashOptions {
   uint32_t dihedral_mode;   /* YUME_PDQ_AMD_DIHEDRAL_* */

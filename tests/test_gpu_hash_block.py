@@ -69,7 +69,7 @@ def colour(imgs, channels):
 
 
 @pytest.mark.parametrize("kind", ["f32", "u8", "rgb", "rgba"])
-@pytest.mark.parametrize("waves", [4, 8])
+@pytest.mark.parametrize("waves", [4, 8, 16])
 def test_block_exact_equals_wave(gpu, dev, imgs, kind, waves):
     import torch
     if kind == "f32":
@@ -78,7 +78,7 @@ def test_block_exact_equals_wave(gpu, dev, imgs, kind, waves):
         x = torch.from_numpy(imgs).to(dev)
     else:
         x = torch.from_numpy(colour(imgs, 3 if kind == "rgb" else 4)).to(dev)
-    layout = gpu.LAYOUT_BLOCK4 if waves == 4 else gpu.LAYOUT_BLOCK8
+    layout = {4: gpu.LAYOUT_BLOCK4, 8: gpu.LAYOUT_BLOCK8, 16: gpu.LAYOUT_BLOCK16}[waves]
     for mode in (0, 1):
         say(f"[block] exact {kind} waves={waves} mode={mode} n={x.shape[0]}")
         a = run(gpu, x, gpu.LAYOUT_WAVE, mode=mode)
@@ -133,7 +133,7 @@ def test_block_edge_planes(gpu, dev):
     for mode in (0, 1):
         say(f"[block] edge planes mode={mode}")
         a = run(gpu, x, gpu.LAYOUT_WAVE, mode=mode)
-        for layout in (gpu.LAYOUT_BLOCK4, gpu.LAYOUT_BLOCK8):
+        for layout in (gpu.LAYOUT_BLOCK4, gpu.LAYOUT_BLOCK8, gpu.LAYOUT_BLOCK16):
             assert_same(a, run(gpu, x, layout, mode=mode), f"edge planes layout={layout} mode={mode}")
         nf = [names.index(k) for k in NONFINITE]
         xn = x[nf]

@@ -310,8 +310,10 @@ int yume_pdq_amd_hash_batch_ex(const void* d_planes, int input_kind, uint64_t pl
   const uint64_t plane_bytes = ypdq::input_plane_bytes(input_kind);
   if (plane_bytes == 0) return YUME_PDQ_AMD_EINVAL;
   if (o.dihedral_mode > YUME_PDQ_AMD_DIHEDRAL_AVX2) return YUME_PDQ_AMD_EINVAL;
-  if (o.dct_precision > YUME_PDQ_AMD_DCT_F16X3 || o.layout > YUME_PDQ_AMD_LAYOUT_BLOCK8) return YUME_PDQ_AMD_EINVAL;
-  if (o.layout == YUME_PDQ_AMD_LAYOUT_WAVE && o.dct_precision != YUME_PDQ_AMD_DCT_EXACT) return YUME_PDQ_AMD_EINVAL;
+  if (o.dct_precision > YUME_PDQ_AMD_DCT_F16X3 || o.layout > YUME_PDQ_AMD_LAYOUT_BLOCK16) return YUME_PDQ_AMD_EINVAL;
+  if ((o.layout == YUME_PDQ_AMD_LAYOUT_WAVE || o.layout == YUME_PDQ_AMD_LAYOUT_BLOCK16) &&
+      o.dct_precision != YUME_PDQ_AMD_DCT_EXACT)
+    return YUME_PDQ_AMD_EINVAL;
   if (plane_stride_bytes < plane_bytes || plane_stride_bytes % 16 || (uintptr_t)d_planes % 16)
     return YUME_PDQ_AMD_EINVAL;
   HashOutputs out;
@@ -324,13 +326,14 @@ int yume_pdq_amd_hash_batch_ex(const void* d_planes, int input_kind, uint64_t pl
   out.dihedral_pdqf = nullptr;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   int layout = (int)o.layout;
-  if (layout == YUME_PDQ_AMD_LAYOUT_AUTO)
-    layout = ypdq::hash_layout_for(n_images, (int)o.dct_precision) == 1 ? YUME_PDQ_AMD_LAYOUT_WAVE
-                                                                       : YUME_PDQ_AMD_LAYOUT_BLOCK8;
+  if (layout == YUME_PDQ_AMD_LAYOUT_AUTO) {
+    const int a = ypdq::hash_layout_for(n_images, (int)o.dct_precision);
+    layout = a == 1 ? YUME_PDQ_AMD_LAYOUT_WAVE : a == 16 ? YUME_PDQ_AMD_LAYOUT_BLOCK16 : YUME_PDQ_AMD_LAYOUT_BLOCK8;
+  }
   if (layout == YUME_PDQ_AMD_LAYOUT_WAVE)
     return from_hip(ypdq::launch_hash_wave(d_planes, input_kind, plane_stride_bytes, n_images, nullptr,
                                            o.dihedral_mode, out, o.luma_coeff, s));
-  const int waves = layout == YUME_PDQ_AMD_LAYOUT_BLOCK8 ? 8 : 4;
+  const int waves = layout == YUME_PDQ_AMD_LAYOUT_BLOCK16 ? 16 : layout == YUME_PDQ_AMD_LAYOUT_BLOCK8 ? 8 : 4;
   return from_hip(ypdq::launch_hash_block(d_planes, input_kind, plane_stride_bytes, n_images, nullptr,
                                           o.dihedral_mode, (int)o.dct_precision, waves, out, o.luma_coeff, s));
 }

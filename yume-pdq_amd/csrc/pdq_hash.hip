@@ -62,24 +62,38 @@ template <int P, int S, int D, typename T, typename Raw>
 __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lane, int i, float (&acc)[16][2][2],
                                             Chains& cur, Chains& nxt, Raw (&pre)[D][4], float* __restrict__ buf1,
                                             const Luma& lc) {
-  const bool more = i + D < 127;
+  // the rows D steps ahead, loaded unconditionally and clamped into the
+  // plane (past row 510 they are not used): a load under a branch makes the
+  // compiler copy the ring's registers and wait for every load in flight
   const int r0 = 4 * (i + D) + 3;
-  Row8 x;
-  to_f32(pre[S][0], x, lc);
-  if (more) load_raw(plane, r0, lane, pre[S][0]);
-  chains_feed<2>(cur, x);
-  to_f32(pre[S][1], x, lc);
-  if (more) load_raw(plane, r0 + 1, lane, pre[S][1]);
-  chains_feed<3>(cur, x);
-  chains_zero(nxt);
-  to_f32(pre[S][2], x, lc);
-  if (more) load_raw(plane, r0 + 2, lane, pre[S][2]);
-  chains_feed<4>(cur, x);
-  chains_feed<0>(nxt, x);
-  to_f32(pre[S][3], x, lc);
-  if (more) load_raw(plane, r0 + 3, lane, pre[S][3]);
-  chains_feed<5>(cur, x);
-  chains_feed<1>(nxt, x);
+  auto row = [&](int d) { return min(r0 + d, 511); };
+  if constexpr (std::is_same<T, float>::value) {
+    // f32 rows feed from their ring slots, which reload after their last use
+    chains_feed<2>(cur, pre[S][0]);
+    load_raw(plane, row(0), lane, pre[S][0]);
+    chains_feed<3>(cur, pre[S][1]);
+    load_raw(plane, row(1), lane, pre[S][1]);
+    chains_feed<4>(cur, pre[S][2]);
+    chains_feed<5>(cur, pre[S][3]);
+    chains_start(nxt, pre[S][2], pre[S][3]);
+    load_raw(plane, row(2), lane, pre[S][2]);
+    load_raw(plane, row(3), lane, pre[S][3]);
+  } else {
+    Row8 x, x2, x3;
+    to_f32(pre[S][0], x, lc);
+    load_raw(plane, row(0), lane, pre[S][0]);
+    chains_feed<2>(cur, x);
+    to_f32(pre[S][1], x, lc);
+    load_raw(plane, row(1), lane, pre[S][1]);
+    chains_feed<3>(cur, x);
+    to_f32(pre[S][2], x2, lc);
+    load_raw(plane, row(2), lane, pre[S][2]);
+    chains_feed<4>(cur, x2);
+    to_f32(pre[S][3], x3, lc);
+    load_raw(plane, row(3), lane, pre[S][3]);
+    chains_feed<5>(cur, x3);
+    chains_start(nxt, x2, x3);
+  }
   float c0, c1;
   chains_finish(cur, c0, c1);
   if (buf1) {
@@ -87,7 +101,6 @@ __device__ __forceinline__ void stream_step(const T* __restrict__ plane, int lan
     buf1[i * 128 + 2 * lane + 1] = (lane == 63) ? 0.0f : c1;
   }
   stage1_acc<P>(acc, c0, c1, kDctT.v + i * 16);
-  cur = nxt;
 }
 
 template <typename T, int D>
@@ -100,8 +113,9 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
 #pragma unroll
     for (int jj = 0; jj < 2; jj++) acc[k][jj][0] = acc[k][jj][1] = 0.0f;
 
-  Chains cur, nxt;
-  chains_zero(cur);
+  // two chain sets that trade roles from step to step (the next row's
+  // chains are the current ones of the step after): no copies
+  Chains a, b;
   Raw pre[D][4];
   {
     Raw r1, r2;
@@ -111,25 +125,24 @@ __device__ __forceinline__ void stream_image(const T* __restrict__ plane, int la
     for (int s = 0; s < D; s++)
 #pragma unroll
       for (int d = 0; d < 4; d++) load_raw(plane, 4 * s + 3 + d, lane, pre[s][d]);
-    Row8 x;
-    to_f32(r1, x, lc);
-    chains_feed<0>(cur, x);
-    to_f32(r2, x, lc);
-    chains_feed<1>(cur, x);
+    Row8 x1, x2;
+    to_f32(r1, x1, lc);
+    to_f32(r2, x2, lc);
+    chains_start(a, x1, x2);
   }
   // The accumulator a compressed row feeds is fixed by i mod 4 (alt), so the
   // loop is unrolled by four to keep every accumulator and ring index a
   // constant.
 #pragma unroll 1
   for (int i = 0; i < 124; i += 4) {
-    stream_step<1, 0 % D, D>(plane, lane, i, acc, cur, nxt, pre, buf1, lc);
-    stream_step<0, 1 % D, D>(plane, lane, i + 1, acc, cur, nxt, pre, buf1, lc);
-    stream_step<0, 2 % D, D>(plane, lane, i + 2, acc, cur, nxt, pre, buf1, lc);
-    stream_step<1, 3 % D, D>(plane, lane, i + 3, acc, cur, nxt, pre, buf1, lc);
+    stream_step<1, 0 % D, D>(plane, lane, i, acc, a, b, pre, buf1, lc);
+    stream_step<0, 1 % D, D>(plane, lane, i + 1, acc, b, a, pre, buf1, lc);
+    stream_step<0, 2 % D, D>(plane, lane, i + 2, acc, a, b, pre, buf1, lc);
+    stream_step<1, 3 % D, D>(plane, lane, i + 3, acc, b, a, pre, buf1, lc);
   }
-  stream_step<1, 124 % D, D>(plane, lane, 124, acc, cur, nxt, pre, buf1, lc);
-  stream_step<0, 125 % D, D>(plane, lane, 125, acc, cur, nxt, pre, buf1, lc);
-  stream_step<0, 126 % D, D>(plane, lane, 126, acc, cur, nxt, pre, buf1, lc);
+  stream_step<1, 124 % D, D>(plane, lane, 124, acc, a, b, pre, buf1, lc);
+  stream_step<0, 125 % D, D>(plane, lane, 125, acc, b, a, pre, buf1, lc);
+  stream_step<0, 126 % D, D>(plane, lane, 126, acc, a, b, pre, buf1, lc);
 }
 
 // Stage-1 rows into LDS: tmp[k][j] = nan0(acc0) + nan0(acc1) (x86.rs:207-224);

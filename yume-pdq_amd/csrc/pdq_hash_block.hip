@@ -66,38 +66,65 @@ template <int S, int D, typename T, typename Raw>
 __device__ __forceinline__ void band_step(const T* __restrict__ plane, int lane, int i, int i_end, Chains& cur,
                                           Chains& nxt, Raw (&pre)[D][4], float* __restrict__ c, const Luma& lc,
                                           bool& bad) {
-  const bool more = i + D < i_end;
+  // The rows D steps ahead are loaded unconditionally, clamped into the
+  // plane (past the band's end they are not used): a load under a branch
+  // makes the compiler copy the ring's registers and wait for every load in
+  // flight.
   const int r0 = 4 * (i + D) + 3;
-  Row8 x;
-  to_f32(pre[S][0], x, lc);
-  if (more) load_raw(plane, r0, lane, pre[S][0]);
-  chains_feed<2>(cur, x);
-  to_f32(pre[S][1], x, lc);
-  if (more) load_raw(plane, r0 + 1, lane, pre[S][1]);
-  chains_feed<3>(cur, x);
-  chains_zero(nxt);
-  to_f32(pre[S][2], x, lc);
-  if (more) load_raw(plane, r0 + 2, lane, pre[S][2]);
-  chains_feed<4>(cur, x);
-  chains_feed<0>(nxt, x);
-  to_f32(pre[S][3], x, lc);
-  if (more) load_raw(plane, r0 + 3, lane, pre[S][3]);
-  chains_feed<5>(cur, x);
-  chains_feed<1>(nxt, x);
+  auto row = [&](int d) { return min(r0 + d, 511); };
+  (void)i_end;
+  if constexpr (std::is_same<T, float>::value) {
+    // f32 rows feed from their ring slots, which reload after their last use
+    // (a copy out of the slot would cost eight moves per row)
+    chains_feed<2>(cur, pre[S][0]);
+    load_raw(plane, row(0), lane, pre[S][0]);
+    chains_feed<3>(cur, pre[S][1]);
+    load_raw(plane, row(1), lane, pre[S][1]);
+    chains_feed<4>(cur, pre[S][2]);
+    chains_feed<5>(cur, pre[S][3]);
+    chains_start(nxt, pre[S][2], pre[S][3]);
+    load_raw(plane, row(2), lane, pre[S][2]);
+    load_raw(plane, row(3), lane, pre[S][3]);
+  } else {
+    // converted rows free their slots at once
+    Row8 x, x2, x3;
+    to_f32(pre[S][0], x, lc);
+    load_raw(plane, row(0), lane, pre[S][0]);
+    chains_feed<2>(cur, x);
+    to_f32(pre[S][1], x, lc);
+    load_raw(plane, row(1), lane, pre[S][1]);
+    chains_feed<3>(cur, x);
+    to_f32(pre[S][2], x2, lc);
+    load_raw(plane, row(2), lane, pre[S][2]);
+    chains_feed<4>(cur, x2);
+    to_f32(pre[S][3], x3, lc);
+    load_raw(plane, row(3), lane, pre[S][3]);
+    chains_feed<5>(cur, x3);
+    chains_start(nxt, x2, x3);
+  }
   float c0, c1;
   chains_finish(cur, c0, c1);
   if (lane == 63) c1 = 0.0f;  // column 127 is the zero pad
   if constexpr (std::is_same<T, float>::value) bad = bad || nonfinite(c0) || nonfinite(c1);
   *reinterpret_cast<float2*>(&c[i * 128 + 2 * lane]) = make_float2(c0, c1);
-  cur = nxt;
 }
 
-template <int S, int D, typename T, typename Raw>
-__device__ __forceinline__ void band_steps(const T* __restrict__ plane, int lane, int i, int i_end, Chains& cur,
-                                           Chains& nxt, Raw (&pre)[D][4], float* __restrict__ c, const Luma& lc,
+// D steps; the two chain sets trade roles from step to step (the next row's
+// chains are the current ones of the step after), so no step copies them.
+// kAll: every step runs (no test, so the ring's registers stay fixed);
+// otherwise the steps up to the band's end.
+template <int S, int D, bool kAll, typename T, typename Raw>
+__device__ __forceinline__ void band_steps(const T* __restrict__ plane, int lane, int i, int i_end, Chains& a,
+                                           Chains& b, Raw (&pre)[D][4], float* __restrict__ c, const Luma& lc,
                                            bool& bad) {
-  if (i + S < i_end) band_step<S, D>(plane, lane, i + S, i_end, cur, nxt, pre, c, lc, bad);
-  if constexpr (S + 1 < D) band_steps<S + 1, D>(plane, lane, i, i_end, cur, nxt, pre, c, lc, bad);
+  static_assert(D % 2 == 0, "an even number of steps returns the chain sets to their roles");
+  if (kAll || i + S < i_end) {
+    if constexpr (S % 2 == 0)
+      band_step<S, D>(plane, lane, i + S, i_end, a, b, pre, c, lc, bad);
+    else
+      band_step<S, D>(plane, lane, i + S, i_end, b, a, pre, c, lc, bad);
+  }
+  if constexpr (S + 1 < D) band_steps<S + 1, D, kAll>(plane, lane, i, i_end, a, b, pre, c, lc, bad);
 }
 
 // Compressed rows [i0, i1) into c; true if one of them is not finite.
@@ -107,7 +134,6 @@ __device__ __forceinline__ bool compress_band(const T* __restrict__ plane, int l
   using Raw = typename RawRow<T>::type;
   constexpr int D = BandPrefetch<T>::D;
   Chains cur, nxt;
-  chains_zero(cur);
   Raw pre[D][4];
   {
     Raw r1, r2;
@@ -115,19 +141,18 @@ __device__ __forceinline__ bool compress_band(const T* __restrict__ plane, int l
     load_raw(plane, 4 * i0 + 2, lane, r2);
 #pragma unroll
     for (int s = 0; s < D; s++)
-      if (i0 + s < i1) {
 #pragma unroll
-        for (int d = 0; d < 4; d++) load_raw(plane, 4 * (i0 + s) + 3 + d, lane, pre[s][d]);
-      }
-    Row8 x;
-    to_f32(r1, x, lc);
-    chains_feed<0>(cur, x);
-    to_f32(r2, x, lc);
-    chains_feed<1>(cur, x);
+      for (int d = 0; d < 4; d++) load_raw(plane, min(4 * (i0 + s) + 3 + d, 511), lane, pre[s][d]);
+    Row8 x1, x2;
+    to_f32(r1, x1, lc);
+    to_f32(r2, x2, lc);
+    chains_start(cur, x1, x2);
   }
   bool bad = false;
+  int i = i0;
 #pragma unroll 1
-  for (int i = i0; i < i1; i += D) band_steps<0, D>(plane, lane, i, i1, cur, nxt, pre, c, lc, bad);
+  for (; i + D <= i1; i += D) band_steps<0, D, true>(plane, lane, i, i1, cur, nxt, pre, c, lc, bad);
+  if (i < i1) band_steps<0, D, false>(plane, lane, i, i1, cur, nxt, pre, c, lc, bad);
   return bad;
 }
 
@@ -365,9 +390,13 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
                                                                      uint64_t plane_stride, uint32_t n_images,
                                                                      const uint32_t* __restrict__ d_n, uint32_t mode,
                                                                      HashOutputs out, Luma lc) {
-  static_assert(kWaves == 4 || kWaves == 8, "four or eight waves per image");
+  static_assert(kWaves == 4 || kWaves == 8 || kWaves == 16, "four, eight or sixteen waves per image");
+  static_assert(kWaves <= 8 || kPrec == kDctExact, "sixteen waves: exact arithmetic only");
   constexpr int kBand = (127 + kWaves - 1) / kWaves;  // compressed rows per wave
-  constexpr int kNK = 16 / (kWaves / 2);              // exact stage 1: frequencies per thread
+  // exact stage 1 on at most eight waves: with two frequencies per thread
+  // its LDS reads of the plane double
+  constexpr int kS1Waves = kWaves > 8 ? 8 : kWaves;
+  constexpr int kNK = 16 / (kS1Waves / 2);  // exact stage 1: frequencies per thread
   const uint32_t n = d_n ? min(*d_n, n_images) : n_images;
   const uint32_t img = blockIdx.x;
   if (img >= n) return;
@@ -411,17 +440,20 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
     // B. exact stage 1: thread (j = t % 128, frequencies kNK * (t / 128) ..)
     const int j = t & 127;
     const int k0 = kNK * __builtin_amdgcn_readfirstlane(t >> 7);
+    const bool s1 = wv < kS1Waves;  // wave-uniform
     float acc[kNK][2];
-    stage1_exact<kNK>(c + j, L.dct_t + k0, acc);
+    if (s1) stage1_exact<kNK>(c + j, L.dct_t + k0, acc);
     __syncthreads();  // every read of the plane is done: T and d2 overwrite it
+    if (s1) {
 #pragma unroll
-    for (int k = 0; k < kNK; k++) {
-      float a0 = acc[k][0], a1 = acc[k][1];
-      a0 = __builtin_isnan(a0) ? 0.0f : a0;
-      a1 = __builtin_isnan(a1) ? 0.0f : a1;
-      W.t[(k0 + k) * kTS + j] = (j == 127) ? 0.0f : a0 + a1;
+      for (int k = 0; k < kNK; k++) {
+        float a0 = acc[k][0], a1 = acc[k][1];
+        a0 = __builtin_isnan(a0) ? 0.0f : a0;
+        a1 = __builtin_isnan(a1) ? 0.0f : a1;
+        W.t[(k0 + k) * kTS + j] = (j == 127) ? 0.0f : a0 + a1;
+      }
     }
-  } else {
+  } else if constexpr (kPrec != kDctExact) {
     // B'. stage 1 on matrix cores: wave w computes column tiles w, w + kWaves
     typename Precision<kPrec>::Op d[4];
     dct_operands<kPrec>(lane, d);
@@ -464,15 +496,15 @@ __global__ void __launch_bounds__(64 * kWaves) pdq_hash_block_kernel(const T* __
     dihedral_pdqf_element(W, idx, d0, kPer, mode);
   }
   __syncthreads();
-  if constexpr (kWaves == 8) {
+  if constexpr (kWaves >= 8) {
     // waves 0..3 search the four thresholds and write the hashes, wave 4
-    // the qualities, waves 5..7 the optional PDQF outputs
+    // the qualities, the others the optional PDQF outputs
     if (wv < 4) {
       tail_rows_thresholds_bits(W, lane, wv, out, img, nullptr);
     } else if (wv == 4) {
       tail_quality(W, lane, out, img, true);
     } else {
-      tail_copy_pdqfs(W, t - 5 * 64, 3 * 64, out, img);
+      tail_copy_pdqfs(W, t - 5 * 64, 64 * (kWaves - 5), out, img);
     }
   } else {
     // the quality on wave 1 while wave 0 searches the thresholds
@@ -515,30 +547,60 @@ hipError_t launch_block_waves(const void* planes, int input_kind, uint64_t plane
   }
 }
 
+// Sixteen waves per image: exact arithmetic only.
+hipError_t launch_block_waves16(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
+                                const uint32_t* d_n, uint32_t mode, int dct, const HashOutputs& out, const Luma& lc,
+                                hipStream_t stream) {
+  if (dct != kDctExact) return hipErrorInvalidValue;
+  switch (input_kind) {
+    case kInputF32: launch_block_kind<float, kDctExact, 16>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputU8: launch_block_kind<uint8_t, kDctExact, 16>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputRgb8: launch_block_kind<Rgb8, kDctExact, 16>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    case kInputRgba8: launch_block_kind<Rgba8, kDctExact, 16>(planes, plane_stride, n_max, d_n, mode, out, lc, stream); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess)
+    return 0;
+  return cus;
+}
+
 }  // namespace
 
 int hash_layout_for(uint32_t n_images, int dct) {
-  // Exact arithmetic: eight waves per image (18 us for one u8 image, 23 us
-  // for 128, 71 us for 1024, against 60 / 71 / 78 us for a wave per image)
-  // until the batch is large enough that a wave per image fills the GPU with
-  // less LDS per image (1536 u8 images: 91 us against 105 us; 4096: 218
-  // against 261 us).  Matrix-core arithmetic exists only in the block form.
-  return (dct == kDctExact && n_images > 1024) ? 1 : 8;
+  // Exact arithmetic: sixteen waves per image while the images fit one per
+  // CU, then eight (18 us for one u8 image, 23 us for 128, 71 us for 1024,
+  // against 60 / 71 / 78 us for a wave per image) until the batch is large
+  // enough that a wave per image fills the GPU with less LDS per image (1536
+  // u8 images: 91 us against 105 us; 4096: 193-208 against 261 us).
+  // Matrix-core arithmetic exists only in the block forms of 4 and 8 waves.
+  if (dct != kDctExact) return 8;
+  if (n_images > 1024) return 1;
+  const int cus = device_cus();
+  return (cus > 0 && n_images <= (uint32_t)cus) ? 16 : 8;
 }
 
 hipError_t launch_hash(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_images, uint32_t mode,
                        const HashOutputs& out, hipStream_t stream, const float* luma) {
-  if (hash_layout_for(n_images, kDctExact) == 1)
+  const int layout = hash_layout_for(n_images, kDctExact);
+  if (layout == 1)
     return launch_hash_wave(planes, input_kind, plane_stride, n_images, nullptr, mode, out, luma, stream);
-  return launch_hash_block(planes, input_kind, plane_stride, n_images, nullptr, mode, kDctExact, 8, out, luma,
+  return launch_hash_block(planes, input_kind, plane_stride, n_images, nullptr, mode, kDctExact, layout, out, luma,
                            stream);
 }
 
 hipError_t launch_hash_device_count(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
                                     const uint32_t* d_n, uint32_t mode, const HashOutputs& out, hipStream_t stream) {
-  if (hash_layout_for(n_max, kDctExact) == 1)
+  const int layout = hash_layout_for(n_max, kDctExact);
+  if (layout == 1)
     return launch_hash_wave(planes, input_kind, plane_stride, n_max, d_n, mode, out, nullptr, stream);
-  return launch_hash_block(planes, input_kind, plane_stride, n_max, d_n, mode, kDctExact, 8, out, nullptr, stream);
+  return launch_hash_block(planes, input_kind, plane_stride, n_max, d_n, mode, kDctExact, layout, out, nullptr,
+                           stream);
 }
 
 hipError_t launch_hash_block(const void* planes, int input_kind, uint64_t plane_stride, uint32_t n_max,
@@ -547,6 +609,7 @@ hipError_t launch_hash_block(const void* planes, int input_kind, uint64_t plane_
   if (n_max == 0) return hipSuccess;
   Luma lc{0.299f, 0.587f, 0.114f};
   if (luma) lc = Luma{luma[0], luma[1], luma[2]};
+  if (waves == 16) return launch_block_waves16(planes, input_kind, plane_stride, n_max, d_n, mode, dct, out, lc, stream);
   if (waves == 8) return launch_block_waves<8>(planes, input_kind, plane_stride, n_max, d_n, mode, dct, out, lc, stream);
   if (waves == 4) return launch_block_waves<4>(planes, input_kind, plane_stride, n_max, d_n, mode, dct, out, lc, stream);
   return hipErrorInvalidValue;

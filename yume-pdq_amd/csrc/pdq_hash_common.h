@@ -219,11 +219,6 @@ struct Chains {
   float s[10];
 };
 
-__device__ __forceinline__ void chains_zero(Chains& c) {
-#pragma unroll
-  for (int n = 0; n < 10; n++) c.s[n] = 0.0f;
-}
-
 // Feed input row `x` as tap row DI to every chain with a non-zero weight there.
 template <int DI>
 __device__ __forceinline__ void chains_feed(Chains& c, const Row8& x) {
@@ -231,6 +226,19 @@ __device__ __forceinline__ void chains_feed(Chains& c, const Row8& x) {
   for (int n = 0; n < 10; n++) {
     const int l = kChainRole[n];
     if (DI >= 1 || l == 4 || l == 5) c.s[n] = __builtin_fmaf(x.v[kChainCol[n]], tw(DI, l), c.s[n]);
+  }
+}
+
+// The chains of a compressed row from its first two input rows (tap rows 0
+// and 1): each accumulator starts from +0 as the reference's do, so the
+// first term is fma(x, w, +0); written as one expression per chain, with no
+// zeroed register for the compiler to materialise.
+__device__ __forceinline__ void chains_start(Chains& c, const Row8& x0, const Row8& x1) {
+#pragma unroll
+  for (int n = 0; n < 10; n++) {
+    const int l = kChainRole[n];
+    const float s0 = (l == 4 || l == 5) ? __builtin_fmaf(x0.v[kChainCol[n]], tw(0, l), 0.0f) : 0.0f;
+    c.s[n] = __builtin_fmaf(x1.v[kChainCol[n]], tw(1, l), s0);
   }
 }
 

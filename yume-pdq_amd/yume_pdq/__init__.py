@@ -20,7 +20,7 @@ MATCH_VALU = 2
 MATCH_MFMA = 4
 # DCT arithmetic and kernel layout of hash_batch (YUME_PDQ_AMD_DCT_*, _LAYOUT_*)
 DCT_EXACT, DCT_BF16, DCT_F16, DCT_BF16X3, DCT_F16X3 = 0, 1, 2, 3, 4
-LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_BLOCK4, LAYOUT_BLOCK8 = 0, 1, 2, 3
+LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_BLOCK4, LAYOUT_BLOCK8, LAYOUT_BLOCK16 = 0, 1, 2, 3, 4
 
 # Orientation order of the 8 hashes per image (YUME_PDQ_AMD_D_*).
 DIHEDRALS = ["NORMAL", "FLOPPED", "ROTATED_180", "FLIPPED", "ROTATED_90_FLOPPED",
@@ -267,8 +267,9 @@ def hash_batch(planes, dihedral_mode=DIHEDRAL_AVX2, want_pdqf=False, stream=None
 
     dct: DCT_EXACT (the reference's FP32 arithmetic, bit-identical) or a
     matrix-core precision (DCT_BF16, DCT_F16, DCT_BF16X3, DCT_F16X3); layout:
-    LAYOUT_AUTO, LAYOUT_WAVE (one wavefront per image), LAYOUT_BLOCK4 or
-    LAYOUT_BLOCK8 (four or eight wavefronts per image).
+    LAYOUT_AUTO, LAYOUT_WAVE (one wavefront per image), LAYOUT_BLOCK4,
+    LAYOUT_BLOCK8 or LAYOUT_BLOCK16 (four, eight or sixteen wavefronts per
+    image; sixteen with exact arithmetic only).
 
     Returns (hashes (n,8,32) u8, quality (n,8) f32, threshold (n,8) f32
     [, pdqf (n,16,16)][, buf1 (n,128,128) compressed planes])."""
