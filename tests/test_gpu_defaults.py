@@ -86,7 +86,7 @@ def test_the_modes_differ_on_these_images(expected, oracle, images):
         assert {1, 2, 6, 7} <= differ <= {1, 2, 5, 6, 7}, differ
 
 
-@pytest.mark.parametrize("layout", ["auto", "wave", "block8", "block4"])
+@pytest.mark.parametrize("layout", ["auto", "wave", "block8", "block16", "block4"])
 @pytest.mark.parametrize("kind", ["u8", "f32"])
 def test_hash_batch_default(gpu, dev, images, expected, layout, kind):
     import torch
@@ -95,7 +95,7 @@ def test_hash_batch_default(gpu, dev, images, expected, layout, kind):
     if kind == "f32":
         planes = planes.to(torch.float32)
     lay = {"auto": gpu.LAYOUT_AUTO, "wave": gpu.LAYOUT_WAVE, "block8": gpu.LAYOUT_BLOCK8,
-           "block4": gpu.LAYOUT_BLOCK4}[layout]
+           "block4": gpu.LAYOUT_BLOCK4, "block16": gpu.LAYOUT_BLOCK16}[layout]
     h, q, t, p = gpu.hash_batch(planes, want_pdqf=True, layout=lay)
     check_hashes(h.cpu(), q.cpu(), t.cpu(), expected)
     assert np.array_equal(p.cpu().numpy().view(np.uint32), expected[3].view(np.uint32))
