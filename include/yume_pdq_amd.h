@@ -456,6 +456,11 @@ typedef struct YumePdqAmdServerConfig {
   int in_order_compute;       /* batches' kernels run one batch after another (uploads overlap) */
   uint32_t autotune_iterations; /* > 0: autotune at start unless choices for the shard set's
                                    shape are cached (yume_pdq_amd_batcher_autotune, _tune_load) */
+  uint64_t scan_chunk_rows;     /* > 0: shared circular scan instead of batches: the shards are
+                                   matched in chunks of this many rows, round and round, against
+                                   every active request's hashes; a request joins at the next chunk
+                                   and leaves after one full turn (up to max_batch_images x
+                                   max_contexts images active) */
 } YumePdqAmdServerConfig;
 typedef struct YumePdqAmdResult {
   int status;                 /* YUME_PDQ_AMD_OK or an error of the batch */

@@ -225,3 +225,89 @@ def test_submit_validation(gpu, pool, served_db):
             srv.wait(12345, timeout_ms=10)
     finally:
         srv.close()
+
+
+def test_scan_alone_mix_and_direct(gpu, dev, pool, served_db):
+    """The shared circular scan (200,704-row chunks: three of the 600k-row
+    shard, one of the 150k-row shard): each request's result is the same
+    alone and joining a scan already under way with other requests, and
+    equals the direct hash + match entry points over the whole database."""
+    import torch
+    shards, full = served_db
+    d_full = torch.from_numpy(full).to(dev)
+    reqs = requests(40, seed=11)
+    srv = gpu.Server(shards, max_batch_images=64, max_contexts=3, threshold=31, scan_chunk_rows=200_000)
+    try:
+        say("[server] scan: requests alone")
+        alone = [srv.wait(srv.submit(pool[idx])) for idx in reqs]
+        say("[server] scan: the same requests from 4 threads at once")
+        ids = [None] * len(reqs)
+
+        def client(k):
+            for j in range(k, len(reqs), 4):
+                ids[j] = srv.submit(pool[reqs[j]])
+
+        th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        mix = [srv.wait(i) for i in ids]
+        say(f"[server] scan stats {srv.stats()}")
+        assert any(m["batch_images"] > len(idx) for m, idx in zip(mix, reqs))  # shared launches
+        for idx, a, m in zip(reqs, alone, mix):
+            assert a["status"] == 0 and m["status"] == 0
+            assert same(a, m)
+        say("[server] scan: against the direct entry points")
+        for idx, a in zip(reqs, alone):
+            h, q, t, hits = direct(gpu, dev, pool, d_full, idx)
+            assert np.array_equal(a["hashes"], h)
+            assert np.array_equal(a["quality"].view(np.uint32), q.view(np.uint32))
+            assert np.array_equal(a["threshold"].view(np.uint32), t.view(np.uint32))
+            assert np.array_equal(triples(a["hits"]), triples(hits))
+    finally:
+        srv.close()
+
+
+def test_scan_pinned_and_small_area(gpu, dev, pool, served_db):
+    """Scan mode with an active area of 64 images (one context of 64): more
+    requests than fit wait for slots in arrival order; zero-copy submissions
+    give the same results as staged ones."""
+    import torch
+    shards, _ = served_db
+    pinned = torch.from_numpy(pool).pin_memory()
+    srv = gpu.Server(shards, max_batch_images=64, max_contexts=1, scan_chunk_rows=100_000)
+    try:
+        say("[server] scan: pinned submit, 64-image area")
+        reqs = requests(16, seed=13)
+        starts = [int(r[0]) % 32 for r in reqs]
+        ids = [srv.submit(pool[s0:s0 + len(r)]) for s0, r in zip(starts, reqs)]
+        ids_p = [srv.submit_pinned(pinned[s0:s0 + len(r)]) for s0, r in zip(starts, reqs)]
+        for i, j in zip(ids, ids_p):
+            a, b = srv.wait(i), srv.wait(j)
+            assert a["status"] == 0 and b["status"] == 0
+            assert same(a, b)
+    finally:
+        srv.close()
+
+
+@pytest.mark.parametrize("rows,cap,chunk", [(50, 10, 1024), (4000, 1 << 17, 4096)])
+def test_scan_hit_overflow_and_prefix(gpu, dev, pool, rows, cap, chunk):
+    """Scan mode at threshold 256 (every pair a hit): a launch with more hits
+    than its buffer is matched again into a larger one (50 rows, 10-hit
+    buffer), and hits past the 65,536 the readback kernel stores come back
+    by a second copy (3 images x 8 x 4000 rows = 96,000 hits in one chunk);
+    the sets are whole."""
+    s = gpu.Shards()
+    s.add(synthetic.random_db(rows, seed=7))
+    srv = gpu.Server(s, max_batch_images=16, threshold=256, hit_capacity=cap, scan_chunk_rows=chunk)
+    try:
+        say(f"[server] scan overflow rows={rows} cap={cap}")
+        r = srv.wait(srv.submit(pool[:3]))
+        assert r["status"] == 0
+        q = r["hits"]["query"].astype(np.int64)
+        i = r["hits"]["index"].astype(np.int64)
+        assert np.array_equal(np.sort(q * rows + i), np.arange(24 * rows))
+    finally:
+        srv.close()
+        s.close()

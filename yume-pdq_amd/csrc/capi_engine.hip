@@ -209,6 +209,7 @@ int yume_pdq_amd_server_create(YumePdqAmdShards* s, const YumePdqAmdServerConfig
   c.busy_poll = cfg->busy_poll != 0;
   c.in_order_compute = cfg->in_order_compute != 0;
   c.autotune_iterations = cfg->autotune_iterations;
+  c.scan_chunk_rows = cfg->scan_chunk_rows;
   const hipError_t e = v->server.start(s->set, c);
   if (e != hipSuccess) {
     delete v;

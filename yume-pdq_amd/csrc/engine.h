@@ -101,6 +101,8 @@ class ShardSet {
   const uint8_t* rows(uint32_t i) const { return i < shards_.size() ? shards_[i].rows : nullptr; }
   uint64_t n_rows(uint32_t i) const { return i < shards_.size() ? shards_[i].n_rows : 0; }
   uint64_t index_base(uint32_t i) const { return i < shards_.size() ? shards_[i].index_base : 0; }
+  // the shard's prefix plane: 16 bytes per row, padded to whole 16-row tiles
+  const uint8_t* plane(uint32_t i) const { return i < shards_.size() ? shards_[i].plane : nullptr; }
 
  private:
   struct Shard {

@@ -38,6 +38,13 @@ struct ServerConfig {
   // context runs the autotuner with this many timed launches per candidate
   // at start (0: use the cached choices or the defaults).
   uint32_t autotune_iterations = 0;
+  // > 0: shared circular scan instead of batches.  The shard set is matched
+  // chunk by chunk (rows per chunk, rounded up to 1024), round and round; a
+  // request's hashes join the active queries at the next chunk and leave
+  // after one full turn, so a request never waits for a batch to finish:
+  // its latency is one turn of the scan over the requests active with it.
+  // Up to max_batch_images x max_contexts images are active at once.
+  uint64_t scan_chunk_rows = 0;
 };
 
 struct RequestResult {
@@ -112,6 +119,59 @@ class Server {
   hipError_t enqueue_batch(Slot& s, bool after_last);
   void harvest(Slot& s);
   void free_slots(uint64_t first, uint32_t n);
+
+  // shared circular scan (scan_chunk_rows > 0)
+  struct ScanChunk {
+    const uint8_t* rows;
+    const uint8_t* plane;
+    uint64_t n_rows;
+    uint64_t index_base;
+  };
+  struct ScanReq {
+    Request r;
+    uint32_t first = 0;          // first image slot of the active area
+    uint32_t launched = 0;       // chunk launches it is part of
+    uint32_t harvested = 0;      // of those, harvested
+    uint64_t first_launch = 0;   // sequence number of its first launch
+    uint32_t images_at_join = 0;  // active images in that launch
+    bool failed = false;
+    std::vector<MatchHit> hits;
+  };
+  struct ScanLaunch {
+    uint32_t chunk = 0;
+    uint32_t nq = 0;  // queries matched: the active area up to its last used slot
+    uint64_t seq = 0;
+    int buf = 0;      // readback and hit buffer pair
+    std::vector<std::shared_ptr<ScanReq>> reqs;  // the requests active in it
+  };
+  hipError_t start_scan();
+  void stop_scan();
+  void loop_scan();
+  bool scan_admit();
+  hipError_t scan_launch();
+  void scan_harvest(ScanLaunch& l);
+  void scan_fail_all();
+  std::vector<ScanChunk> chunks_;
+  uint32_t scan_cap_ = 0;
+  std::vector<uint8_t> scan_used_;  // per image slot of the active area
+  std::vector<std::shared_ptr<ScanReq>> active_;
+  std::deque<ScanLaunch> inflight_;
+  uint32_t next_chunk_ = 0;
+  uint64_t launch_seq_ = 0;
+  hipStream_t scan_stream_ = nullptr;
+  hipEvent_t scan_uploaded_ = nullptr;
+  hipEvent_t scan_done_[2] = {nullptr, nullptr};
+  uint8_t* d_img_ = nullptr;   // active area: planes
+  uint8_t* d_q_ = nullptr;     // their hashes, the queries (8 x 32 bytes per image)
+  float* d_qual_ = nullptr;    // 8 per image
+  float* d_thr_ = nullptr;     // 8 per image
+  uint8_t* h_out_ = nullptr;   // page-locked copy of hashes, qualities, thresholds
+  void* dh_out_ = nullptr;
+  MatchHit* d_shits_[2] = {nullptr, nullptr};
+  unsigned long long* d_scount_[2] = {nullptr, nullptr};
+  uint8_t* h_rb_[2] = {nullptr, nullptr};  // page-locked: count, then the first hits
+  void* dh_rb_[2] = {nullptr, nullptr};
+  uint64_t scan_bytes_ = 0;
 
   ServerConfig cfg_;
   std::shared_ptr<ShardSet> shards_;

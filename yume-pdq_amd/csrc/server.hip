@@ -52,7 +52,7 @@ static_assert(sizeof(MatchHit) == 16, "one hit per 16-byte word");
 Server::~Server() { stop(); }
 
 uint64_t Server::device_bytes() const {
-  uint64_t t = 0;
+  uint64_t t = scan_bytes_;
   for (auto& s : ctx_) t += s->ctx->device_bytes();
   return t;
 }
@@ -65,6 +65,21 @@ hipError_t Server::start(std::shared_ptr<ShardSet> shards, const ServerConfig& c
   cfg_ = cfg;
   shards_ = std::move(shards);
   hipError_t e;
+  if (cfg.scan_chunk_rows > 0) {
+    if ((e = start_scan()) != hipSuccess) {
+      stop_scan();
+      return e;
+    }
+    if ((e = hipStreamCreateWithFlags(&upload_, hipStreamNonBlocking)) != hipSuccess) return e;
+    ring_slots_ = 2ull * scan_cap_ + cfg.max_batch_images;
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&staging_), ring_slots_ * slot_bytes_, hipHostMallocDefault)) !=
+        hipSuccess)
+      return e;
+    slot_used_.assign(ring_slots_, 0);
+    stop_ = false;
+    worker_ = std::thread([this] { loop_scan(); });
+    return hipSuccess;
+  }
   for (uint32_t k = 0; k < std::min<uint32_t>(cfg.max_contexts, 16); k++) {
     std::unique_ptr<Slot> s(new Slot());
     s->ctx.reset(new BatchContext());
@@ -113,6 +128,8 @@ void Server::stop() {
   }
   cv_submit_.notify_all();
   if (worker_.joinable()) worker_.join();
+  stop_scan();
+  scan_bytes_ = 0;
   for (auto& s : ctx_) {
     if (s->done) (void)hipEventDestroy(s->done);
     if (s->computed) (void)hipEventDestroy(s->computed);

@@ -55,7 +55,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("max_batch_images", C.c_uint32), ("hbm_budget_bytes", C.c_uint64), ("max_contexts", C.c_uint32),
                 ("threshold", C.c_uint32), ("dihedral_mode", C.c_uint32), ("hit_capacity", C.c_uint64),
                 ("use_graph", C.c_int), ("busy_poll", C.c_int), ("in_order_compute", C.c_int),
-                ("autotune_iterations", C.c_uint32)]
+                ("autotune_iterations", C.c_uint32), ("scan_chunk_rows", C.c_uint64)]
 
 
 class ServerResult(C.Structure):
@@ -849,10 +849,10 @@ class Server:
 
     def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
                  dihedral_mode=DIHEDRAL_AVX2, hit_capacity=1 << 20, use_graph=True, busy_poll=True,
-                 in_order_compute=True, autotune_iterations=0):
+                 in_order_compute=True, autotune_iterations=0, scan_chunk_rows=0):
         cfg = ServerConfig(int(max_batch_images), int(hbm_budget_bytes), int(max_contexts), int(threshold),
                            int(dihedral_mode), int(hit_capacity), 1 if use_graph else 0, 1 if busy_poll else 0,
-                           1 if in_order_compute else 0, int(autotune_iterations))
+                           1 if in_order_compute else 0, int(autotune_iterations), int(scan_chunk_rows))
         h = C.c_void_p()
         _check(lib().yume_pdq_amd_server_create(shards._h, C.byref(cfg), C.byref(h)), "server_create")
         self._h = h
