@@ -420,7 +420,7 @@ def main():
             "served_images_per_s": top100["served_images_per_s"], "latency_ms": top100["latency_ms"],
             "served_vs_plain": top100["served_vs_plain"],
             "note": ("the highest rate served at >= 95 % of offered on the 100M-row database of the serving "
-                     "configuration; one 64-image request there needs about 2.4 ms of matching, so its p99 stays "
+                     "configuration; one 64-image request there needs about 2 ms of matching, so its p99 stays "
                      "above 2 ms")}
         del d_db100
         torch.cuda.empty_cache()
@@ -516,11 +516,13 @@ def plain_host_rate(yume_pdq, torch, dev, d_db, pool_u8, thr, n=60):
 
 
 def serving_section(args, yume_pdq, torch, dev, synthetic, d_db, db_rows, pool_u8, thr, offered, label,
-                    pool_hashes=None, clients=8, max_batch=64, max_contexts=4, in_order_compute=True):
+                    pool_hashes=None, clients=8, max_batch=64, max_contexts=4, in_order_compute=True,
+                    scan_chunk_rows=0):
     """Open-loop Poisson arrivals at `offered` images/s of requests carrying
-    U{1..64} u8 512x512 images from host memory, served by the continuous-
-    batching server (hash + 8-dihedral match against every shard); latency is
-    submit -> results on the host, PCIe included."""
+    U{1..64} u8 512x512 images from host memory, served by the server (hash +
+    8-dihedral match against every shard; continuous batching, or with
+    scan_chunk_rows > 0 the shared circular scan); latency is submit ->
+    results on the host, PCIe included."""
     import threading
 
     import numpy as np
@@ -534,7 +536,7 @@ def serving_section(args, yume_pdq, torch, dev, synthetic, d_db, db_rows, pool_u
     shards = yume_pdq.Shards()
     shards.add(d_db)  # the rows stay where they are; the set adds the prefix plane
     srv = yume_pdq.Server(shards, max_batch_images=max_batch, max_contexts=max_contexts, hbm_budget_bytes=16 << 30,
-                          threshold=thr, in_order_compute=in_order_compute)
+                          threshold=thr, in_order_compute=in_order_compute, scan_chunk_rows=scan_chunk_rows)
     rng = np.random.default_rng(23)
     qps = offered / 32.5
     gaps = rng.exponential(1.0 / qps, size=int(qps * args.serve_seconds * 1.2) + 10)
@@ -591,6 +593,7 @@ def serving_section(args, yume_pdq, torch, dev, synthetic, d_db, db_rows, pool_u
     served = images / span
     out = {
         "database": label,
+        "mode": f"shared scan, {scan_chunk_rows}-row chunks" if scan_chunk_rows else "batches",
         "db_rows": int(n_rows),
         "requests": int(arrivals.size),
         "images": images,
@@ -606,6 +609,8 @@ def serving_section(args, yume_pdq, torch, dev, synthetic, d_db, db_rows, pool_u
         "plain_unbatched_host_images_per_s": round(plain, 1),
         "served_vs_plain": round(served / plain, 2),
         "client_lag_ms_p99": round(float(np.percentile(lag, 99) * 1e3), 3),
+        "server_latency_ms": {"p50": round(float(np.percentile([r["latency_ms"] for r in res], 50)), 3),
+                              "p99": round(float(np.percentile([r["latency_ms"] for r in res], 99)), 3)},
         "note": ("latency from the scheduled arrival to results on the host; request planes in "
                  "page-locked host memory, uploaded by the server (PCIe included); plain = one image per "
                  "call from the same memory: upload, hash launch, 8-query scan of the raw rows, count read"),

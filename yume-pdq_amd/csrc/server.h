@@ -139,10 +139,11 @@ class Server {
   };
   struct ScanLaunch {
     uint32_t chunk = 0;
-    uint32_t nq = 0;  // queries matched: the active area up to its last used slot
+    uint32_t nq = 0;  // queries matched: every active request's, gathered
     uint64_t seq = 0;
-    int buf = 0;      // readback and hit buffer pair
+    int buf = 0;      // gathered queries, readback and hit buffers
     std::vector<std::shared_ptr<ScanReq>> reqs;  // the requests active in it
+    std::vector<uint32_t> at;                    // first gathered image of each
   };
   hipError_t start_scan();
   void stop_scan();
@@ -167,6 +168,9 @@ class Server {
   float* d_thr_ = nullptr;     // 8 per image
   uint8_t* h_out_ = nullptr;   // page-locked copy of hashes, qualities, thresholds
   void* dh_out_ = nullptr;
+  uint8_t* d_lq_[2] = {nullptr, nullptr};  // a launch's gathered queries
+  uint4* h_tab_[2] = {nullptr, nullptr};    // page-locked gather table: {first slot, first image, images}
+  void* dh_tab_[2] = {nullptr, nullptr};
   MatchHit* d_shits_[2] = {nullptr, nullptr};
   unsigned long long* d_scount_[2] = {nullptr, nullptr};
   uint8_t* h_rb_[2] = {nullptr, nullptr};  // page-locked: count, then the first hits

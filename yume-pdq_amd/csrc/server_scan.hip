@@ -11,13 +11,14 @@
 //
 // One worker thread owns every GPU call.  Chunk launches run on one stream,
 // two in flight: the next chunk's launch is queued before the previous one
-// is harvested.  Each launch is the single-database matcher over the chunk
-// (rows, prefix plane, global index base) with the active area's queries up
-// to its last used slot; a kernel then stores the hit count and the first
-// hits into page-locked memory.  A launch records which requests were
-// active in it, so a hit of a slot that was free, or has since changed
-// hands, is dropped.  Slots are freed only when their request's last launch
-// has been harvested, after which no queued launch reads them.
+// is harvested.  Each launch gathers the hashes of the requests that have
+// not met the chunk yet into one contiguous query array (the active area
+// has holes as requests come and go), runs the single-database matcher over
+// the chunk (rows, prefix plane, global index base) and stores the hit
+// count and the first hits into page-locked memory.  A launch records which
+// request each gathered query belongs to.  Slots are freed only when their
+// request's last launch has been harvested, after which no queued launch
+// reads them.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -45,6 +46,16 @@ __global__ void scan_readback_kernel(const uint4* __restrict__ hits, const unsig
   for (uint64_t i = i0; i < n; i += (uint64_t)gridDim.x * blockDim.x) dst[1 + i] = hits[i];
 }
 
+// Block b copies the hashes of table entry b (first slot, first gathered
+// image, images; 16 16-byte words per image) from the active area to the
+// launch's query array.
+__global__ void scan_gather_kernel(const uint4* __restrict__ area, uint4* __restrict__ dst,
+                                   const uint4* __restrict__ tab) {
+  const uint4 t = tab[blockIdx.x];
+  const uint32_t n16 = t.z * 16;
+  for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) dst[t.y * 16 + i] = area[t.x * 16 + i];
+}
+
 // dst[i] = src[i] for n 16-byte words (device to page-locked host memory)
 __global__ void scan_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -68,7 +79,7 @@ hipError_t Server::start_scan() {
   if (chunks_.empty()) return hipErrorInvalidValue;
   scan_cap_ = std::min<uint32_t>(cfg_.max_batch_images * std::min<uint32_t>(cfg_.max_contexts, 16), 4096);
   const uint64_t hit_bytes = cfg_.hit_capacity * sizeof(MatchHit);
-  scan_bytes_ = scan_cap_ * (kPlane + kOutPerImage) + 2 * (hit_bytes + 256);
+  scan_bytes_ = scan_cap_ * (kPlane + kOutPerImage + 2 * kHashBytes) + 2 * (hit_bytes + 256);
   if (scan_bytes_ > cfg_.hbm_budget_bytes) return hipErrorOutOfMemory;
   if ((e = hipMalloc(&d_img_, scan_cap_ * kPlane)) != hipSuccess) return e;
   if ((e = hipMalloc(&d_q_, scan_cap_ * kHashBytes)) != hipSuccess) return e;
@@ -79,6 +90,11 @@ hipError_t Server::start_scan() {
       (e = hipHostGetDevicePointer(&dh_out_, h_out_, 0)) != hipSuccess)
     return e;
   for (int b = 0; b < 2; b++) {
+    if ((e = hipMalloc(&d_lq_[b], scan_cap_ * kHashBytes)) != hipSuccess) return e;
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_tab_[b]), scan_cap_ * sizeof(uint4), hipHostMallocDefault)) !=
+            hipSuccess ||
+        (e = hipHostGetDevicePointer(&dh_tab_[b], h_tab_[b], 0)) != hipSuccess)
+      return e;
     if ((e = hipMalloc(&d_shits_[b], hit_bytes)) != hipSuccess) return e;
     if ((e = hipMalloc(&d_scount_[b], 256)) != hipSuccess) return e;
     const uint64_t rb = (1 + std::min<uint64_t>(kScanHitPrefix, cfg_.hit_capacity)) * 16;
@@ -95,10 +111,10 @@ hipError_t Server::start_scan() {
 
 void Server::stop_scan() {
   if (scan_stream_) (void)hipStreamSynchronize(scan_stream_);
-  void* d[] = {d_img_, d_q_, d_qual_, d_thr_, d_shits_[0], d_shits_[1], d_scount_[0], d_scount_[1]};
+  void* d[] = {d_img_, d_q_, d_qual_, d_thr_, d_lq_[0], d_lq_[1], d_shits_[0], d_shits_[1], d_scount_[0], d_scount_[1]};
   for (void* p : d)
     if (p) (void)hipFree(p);
-  void* h[] = {h_out_, h_rb_[0], h_rb_[1]};
+  void* h[] = {h_out_, h_rb_[0], h_rb_[1], h_tab_[0], h_tab_[1]};
   for (void* p : h)
     if (p) (void)hipHostFree(p);
   for (hipEvent_t ev : {scan_done_[0], scan_done_[1], scan_uploaded_})
@@ -109,6 +125,9 @@ void Server::stop_scan() {
   h_out_ = nullptr;
   dh_out_ = nullptr;
   for (int b = 0; b < 2; b++) {
+    d_lq_[b] = nullptr;
+    h_tab_[b] = nullptr;
+    dh_tab_[b] = nullptr;
     d_shits_[b] = nullptr;
     d_scount_[b] = nullptr;
     h_rb_[b] = nullptr;
@@ -193,11 +212,13 @@ hipError_t Server::scan_launch() {
   l.seq = ++launch_seq_;
   l.buf = (int)(l.seq & 1);
   const uint32_t n_chunks = (uint32_t)chunks_.size();
-  uint32_t hi = 0, images = 0;
+  // the gather table (its buffer's previous launch has been harvested)
+  uint32_t images = 0;
   for (auto& q : active_)
     if (q->launched < n_chunks) {
+      h_tab_[l.buf][l.reqs.size()] = make_uint4(q->first, images, q->r.n, 0u);
       l.reqs.push_back(q);
-      hi = std::max(hi, q->first + q->r.n);
+      l.at.push_back(images);
       images += q->r.n;
     }
   for (auto& q : l.reqs) {
@@ -206,9 +227,13 @@ hipError_t Server::scan_launch() {
       q->images_at_join = images;
     }
   }
-  l.nq = hi * 8;
+  l.nq = images * 8;
+  hipLaunchKernelGGL(scan_gather_kernel, dim3((uint32_t)l.reqs.size()), dim3(256), 0, scan_stream_,
+                     reinterpret_cast<const uint4*>(d_q_), reinterpret_cast<uint4*>(d_lq_[l.buf]),
+                     static_cast<const uint4*>(dh_tab_[l.buf]));
+  if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
   const ScanChunk& c = chunks_[l.chunk];
-  const int rc = yume_pdq_amd_match_with_plane(c.rows, c.plane, c.n_rows, d_q_, l.nq, cfg_.threshold,
+  const int rc = yume_pdq_amd_match_with_plane(c.rows, c.plane, c.n_rows, d_lq_[l.buf], l.nq, cfg_.threshold,
                                                reinterpret_cast<YumePdqAmdHit*>(d_shits_[l.buf]), cfg_.hit_capacity,
                                                d_scount_[l.buf], c.index_base, 0, scan_stream_);
   if (rc != YUME_PDQ_AMD_OK) return hipErrorLaunchFailure;
@@ -242,16 +267,16 @@ void Server::scan_harvest(ScanLaunch& l) {
     if (e == hipSuccess) e = hipStreamSynchronize(upload_);
     hits = more.data();
   } else if (total > cfg_.hit_capacity) {
-    // the chunk matched again into a buffer that holds every hit: the
-    // launch's requests still hold their slots, and hits of other slots are
-    // dropped below
+    // the chunk matched again into a buffer that holds every hit, from the
+    // launch's gathered queries (the next launch gathers into the other
+    // buffer)
     const ScanChunk& c = chunks_[l.chunk];
     MatchHit* d_big = nullptr;
     unsigned long long* d_cnt = nullptr;
     e = hipMalloc(&d_big, total * sizeof(MatchHit));
     if (e == hipSuccess) e = hipMalloc(&d_cnt, 8);
     if (e == hipSuccess &&
-        yume_pdq_amd_match_with_plane(c.rows, c.plane, c.n_rows, d_q_, l.nq, cfg_.threshold,
+        yume_pdq_amd_match_with_plane(c.rows, c.plane, c.n_rows, d_lq_[l.buf], l.nq, cfg_.threshold,
                                       reinterpret_cast<YumePdqAmdHit*>(d_big), total, d_cnt, c.index_base, 0,
                                       upload_) != YUME_PDQ_AMD_OK)
       e = hipErrorLaunchFailure;
@@ -264,18 +289,14 @@ void Server::scan_harvest(ScanLaunch& l) {
     if (d_cnt) (void)hipFree(d_cnt);
     hits = more.data();
   }
-  // the request of each slot in this launch
-  std::vector<int> owner(scan_cap_, -1);
-  for (size_t i = 0; i < l.reqs.size(); i++)
-    for (uint32_t k = 0; k < l.reqs[i]->r.n; k++) owner[l.reqs[i]->first + k] = (int)i;
   if (e == hipSuccess) {
     for (uint64_t h = 0; h < total; h++) {
-      const uint32_t slot = hits[h].query / 8;
-      if (slot >= scan_cap_ || owner[slot] < 0) continue;  // not active in this launch
-      ScanReq& q = *l.reqs[owner[slot]];
+      // the request whose gathered images hold the query's image
+      const uint32_t img = hits[h].query / 8;
+      const size_t i = (size_t)(std::upper_bound(l.at.begin(), l.at.end(), img) - l.at.begin()) - 1;
       MatchHit x = hits[h];
-      x.query -= 8 * q.first;
-      q.hits.push_back(x);
+      x.query -= 8 * l.at[i];
+      l.reqs[i]->hits.push_back(x);
     }
   }
   const uint32_t n_chunks = (uint32_t)chunks_.size();

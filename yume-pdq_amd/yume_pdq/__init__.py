@@ -845,7 +845,12 @@ def tune_clear():
 class Server:
     """Continuous-batching server over a shard set: submit() u8 planes from
     any thread, wait() for each request's hashes, qualities, thresholds and
-    hits (queries numbered 8 * image + dihedral within the request)."""
+    hits (queries numbered 8 * image + dihedral within the request).
+
+    scan_chunk_rows > 0 selects the shared circular scan instead of batches:
+    the shard set is matched chunk by chunk, round and round, against every
+    active request's hashes; a request joins at the next chunk and is done
+    after one full turn. Results are the same in both modes."""
 
     def __init__(self, shards, max_batch_images=64, hbm_budget_bytes=8 << 30, max_contexts=4, threshold=31,
                  dihedral_mode=DIHEDRAL_AVX2, hit_capacity=1 << 20, use_graph=True, busy_poll=True,
