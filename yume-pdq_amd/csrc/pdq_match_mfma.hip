@@ -796,85 +796,167 @@ __device__ __forceinline__ void resolve_flush(ResolveLds& L, MatchHit* __restric
     if (base + i < capacity) hits[base + i] = L.stage[i];
 }
 
-// One record, checked by one wave: sub-chunk rows row0 ... of a database of
-// n_rows rows (global index index_base + row), queries qbase ...
-// (QT x 16), pending groups of kGroupTiles tiles.  Its loads are issued
-// together (the query words, then each pending group's tile words).
+// The loads of one record part that do not depend on its products: the
+// query words of its tiles and the tile words of its first pending group.  A
+// wave issues them for its next part before it checks the current one, so a
+// part's check waits only on its own exact-distance loads.
+template <int QT>
+struct PartLoads {
+  uint32_t qw[QT];
+  uint32_t w[kGroupTiles];
+};
+
+// Tiles of the sub-chunk at row0 (16 rows each; the last sub-chunk's may be
+// partial).
+__device__ __forceinline__ uint32_t sub_tiles(uint64_t n_rows, uint64_t row0) {
+  const uint64_t left = n_rows - row0;
+  return left >= kSharedRows ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
+}
+
+// Group g's tile words (lane rn, word kq of each tile), clamped into the
+// sub-chunk's tiles (and, for raw rows, its rows).
+template <bool kPlane>
+__device__ __forceinline__ void load_group(uint32_t (&w)[kGroupTiles], const uint32_t* __restrict__ db,
+                                           const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t row0,
+                                           uint32_t g) {
+  const int lane = threadIdx.x & 63;
+  const int rn = lane & 15, kq = lane >> 4;
+  const uint64_t left = n_rows - row0;
+  const uint32_t tiles = sub_tiles(n_rows, row0);
+  const RowSource<kPlane> src(db, plane, row0, lane);
+#pragma unroll
+  for (int j = 0; j < kGroupTiles; j++) {
+    const uint32_t tt = min(kGroupTiles * g + (uint32_t)j, tiles - 1);
+    if (kPlane) {
+      w[j] = src(tt);
+    } else {
+      const uint64_t rr = min((uint64_t)tt * 16 + rn, left - 1);
+      w[j] = db[(row0 + rr) * 8 + kq];
+    }
+  }
+}
+
 template <int QT, bool kPlane>
-__device__ __forceinline__ void resolve_record(ResolveLds& L, const uint32_t* __restrict__ db,
+__device__ __forceinline__ void part_load(PartLoads<QT>& P, const uint32_t* __restrict__ db,
+                                          const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t row0,
+                                          uint32_t qbase, uint32_t pending, const uint32_t* __restrict__ queries,
+                                          uint32_t n_queries) {
+  if (pending == 0) return;  // nothing to check (the same for the whole wave)
+  const int lane = threadIdx.x & 63;
+  const int rn = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < QT; t++) {
+    const uint32_t qi = qbase + 16 * t + rn;
+    P.qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+  }
+  load_group<kPlane>(P.w, db, plane, n_rows, row0, (uint32_t)__builtin_ctz(pending));
+}
+
+// One pending group g of a record part: its tiles' products with the part's
+// query tiles qa, and the pairs of the query tiles that reach the bound
+// checked exactly.  w holds the group's tile words.
+template <int QT>
+__device__ __forceinline__ void check_group(ResolveLds& L, const Expanded (&qa)[QT], uint32_t g,
+                                            const uint32_t (&words)[kGroupTiles], uint32_t tiles, uint64_t left,
+                                            uint64_t row0, uint32_t qbase, float bound,
+                                            const uint32_t* __restrict__ queries, const uint32_t* __restrict__ db,
+                                            uint32_t n_queries, uint64_t n_rows, uint32_t threshold,
+                                            MatchHit* __restrict__ hits, uint64_t capacity,
+                                            unsigned long long* __restrict__ counter, uint64_t index_base) {
+  const int lane = threadIdx.x & 63;
+  const int rn = lane & 15, kq = lane >> 4;
+  const int ibound = __float_as_int(bound);
+  // tile j's word is w[0] in iteration j (the words move down one place per
+  // tile: a rolled loop cannot index registers, and a select by j becomes an
+  // indexed copy in scratch memory)
+  uint32_t w[kGroupTiles];
+#pragma unroll
+  for (int k = 0; k < kGroupTiles; k++) w[k] = words[k];
+#pragma unroll 1
+  for (int j = 0; j < kGroupTiles; j++) {
+    const uint32_t tt = kGroupTiles * g + (uint32_t)j;
+    if (tt >= tiles) break;  // the same for the whole wave
+    const uint32_t wj = w[0];
+#pragma unroll
+    for (int k = 0; k + 1 < kGroupTiles; k++) w[k] = w[k + 1];
+    Expanded eg;
+    expand_word(wj, L.lut, eg);
+    if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
+    // which query tiles have a pair at the bound (a wave-uniform mask); the
+    // products go in pairs, so that one product's reads overlap the next's
+    // matrix-core latency instead of each waiting on the last
+    uint32_t tmask = 0;
+#pragma unroll
+    for (int t = 0; t < QT; t += 2) {
+      const v4f c0 = mfma_prefix(qa[t], eg);
+      const v4f c1 = t + 1 < QT ? mfma_prefix(qa[t + 1 < QT ? t + 1 : t], eg) : c0;
+      const int m0 = max(max(__float_as_int(c0.x), __float_as_int(c0.y)), max(__float_as_int(c0.z), __float_as_int(c0.w)));
+      const int m1 = max(max(__float_as_int(c1.x), __float_as_int(c1.y)), max(__float_as_int(c1.z), __float_as_int(c1.w)));
+      tmask |= (wave_any(m0 >= ibound) ? 1u : 0u) << t;
+      if (t + 1 < QT) tmask |= (wave_any(m1 >= ibound) ? 1u : 0u) << (t + 1);
+    }
+    // only those tiles' products are recomputed and checked, by one inlined
+    // check in a loop over the mask: QT inlined copies keep several hundred
+    // registers live, and an out-of-line check saves and restores registers
+    // through scratch memory at every call
+    while (tmask) {
+      const int t = __builtin_ctz(tmask);
+      tmask &= tmask - 1;
+      Expanded a = qa[0];  // qa[t], selected (registers cannot be indexed)
+#pragma unroll
+      for (int k = 1; k < QT; k++)
+        if (t == k) a = qa[k];
+      stage_candidates(mfma_prefix(a, eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
+                       n_queries, n_rows, threshold, L.stage, &L.n_stage, hits, capacity, counter, index_base);
+    }
+  }
+}
+
+// One record part, checked by one wave: sub-chunk rows row0 ... of a
+// database of n_rows rows (global index index_base + row), queries qbase ...
+// (QT x 16), pending groups of kGroupTiles tiles, with P holding its
+// part_load.  Groups after the first load their tile words here.  `next` is
+// called once P's words are taken (it issues the wave's next loads): a wait
+// for P then never covers those newer loads.
+template <int QT, bool kPlane, class Next>
+__device__ __forceinline__ void resolve_record(ResolveLds& L, const PartLoads<QT>& P,
+                                               const uint32_t* __restrict__ db,
                                                const uint32_t* __restrict__ plane, uint64_t n_rows, uint64_t row0,
                                                uint32_t qbase, uint32_t pending, const uint32_t* __restrict__ queries,
                                                uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
                                                uint64_t capacity, unsigned long long* __restrict__ counter,
-                                               uint64_t index_base) {
-  if (pending == 0) return;  // a half of a record with no candidates
-  const int lane = threadIdx.x & 63;
-  const int rn = lane & 15, kq = lane >> 4;
-  const float bound = (float)(128 - 2 * (int)threshold);
-  const int ibound = __float_as_int(bound);
-  const uint64_t left = n_rows - row0;
-  const uint32_t tiles = left >= kSharedRows ? (uint32_t)kSharedTiles : (uint32_t)((left + 15) / 16);
-  uint32_t qw[QT];
-#pragma unroll
-  for (int t = 0; t < QT; t++) {
-    const uint32_t qi = qbase + 16 * t + rn;
-    qw[t] = queries[(size_t)(qi < n_queries ? qi : n_queries - 1) * 8 + kq];
+                                               uint64_t index_base, Next&& next) {
+  if (pending == 0) {  // a half of a record with no candidates
+    next();
+    return;
   }
+  const int rn = threadIdx.x & 15;
+  const float bound = (float)(128 - 2 * (int)threshold);
+  const uint64_t left = n_rows - row0;
+  const uint32_t tiles = sub_tiles(n_rows, row0);
   Expanded qa[QT];
 #pragma unroll
   for (int t = 0; t < QT; t++) {
-    expand_word(qw[t], L.lut, qa[t]);
+    expand_word(P.qw[t], L.lut, qa[t]);
     if (qbase + 16 * t + rn >= n_queries) qa[t].d[0] = qa[t].d[1] = qa[t].d[2] = qa[t].d[3] = 0;
   }
-  const RowSource<kPlane> src(db, plane, row0, lane);
-  while (pending) {
+  // The first group's words come with P, outside the loop over the other
+  // groups: where the two meet, a wait for the loop's own loads would also
+  // wait for the loads `next` has issued.
+  uint32_t w0[kGroupTiles];
+#pragma unroll
+  for (int j = 0; j < kGroupTiles; j++) w0[j] = P.w[j];
+  next();
+  check_group<QT>(L, qa, (uint32_t)__builtin_ctz(pending), w0, tiles, left, row0, qbase, bound, queries, db,
+                  n_queries, n_rows, threshold, hits, capacity, counter, index_base);
+  pending &= pending - 1;
+  while (pending) {  // rare: a part with candidates in more than one group
     const uint32_t g = (uint32_t)__builtin_ctz(pending);
     pending &= pending - 1;
     uint32_t w[kGroupTiles];
-#pragma unroll
-    for (int j = 0; j < kGroupTiles; j++) {
-      // clamped into the sub-chunk's tiles (and, for raw rows, its rows)
-      const uint32_t tt = min(kGroupTiles * g + (uint32_t)j, tiles - 1);
-      if (kPlane) {
-        w[j] = src(tt);
-      } else {
-        const uint64_t rr = min((uint64_t)tt * 16 + rn, left - 1);
-        w[j] = db[(row0 + rr) * 8 + kq];
-      }
-    }
-#pragma unroll 1
-    for (int j = 0; j < kGroupTiles; j++) {
-      const uint32_t tt = kGroupTiles * g + (uint32_t)j;
-      if (tt >= tiles) break;  // the same for the whole wave
-      uint32_t wj = w[0];  // w[j], selected (a rolled loop cannot index registers)
-#pragma unroll
-      for (int k = 1; k < kGroupTiles; k++) wj = j == k ? w[k] : wj;
-      Expanded eg;
-      expand_word(wj, L.lut, eg);
-      if ((uint64_t)tt * 16 + rn >= left) eg.d[0] = eg.d[1] = eg.d[2] = eg.d[3] = 0;
-      // which query tiles have a pair at the bound (a wave-uniform mask)
-      uint32_t tmask = 0;
-#pragma unroll
-      for (int t = 0; t < QT; t++) {
-        const v4f c = mfma_prefix(qa[t], eg);
-        const int m = max(max(__float_as_int(c.x), __float_as_int(c.y)), max(__float_as_int(c.z), __float_as_int(c.w)));
-        tmask |= (wave_any(m >= ibound) ? 1u : 0u) << t;
-      }
-      // only those tiles' products are recomputed and checked, by one
-      // inlined check in a loop over the mask: QT inlined copies keep
-      // several hundred registers live, and an out-of-line check saves and
-      // restores registers through scratch memory at every call
-      while (tmask) {
-        const int t = __builtin_ctz(tmask);
-        tmask &= tmask - 1;
-        Expanded a = qa[0];  // qa[t], selected (registers cannot be indexed)
-#pragma unroll
-        for (int k = 1; k < QT; k++)
-          if (t == k) a = qa[k];
-        stage_candidates(mfma_prefix(a, eg), t, row0 + (uint64_t)tt * 16, qbase, rn, kq, bound, queries, db,
-                         n_queries, n_rows, threshold, L.stage, &L.n_stage, hits, capacity, counter, index_base);
-      }
-    }
+    load_group<kPlane>(w, db, plane, n_rows, row0, g);
+    check_group<QT>(L, qa, g, w, tiles, left, row0, qbase, bound, queries, db, n_queries, n_rows, threshold, hits,
+                    capacity, counter, index_base);
   }
 }
 
@@ -923,20 +1005,51 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_kernel(
     (void)groups;
     return;
   }
-  // item r: part r % kParts of record r / kParts
+  // The wave's records are w, w + S, w + 2S, ... (w its number in the
+  // grid, S the grid's waves); its item j is part j % kParts of the
+  // (j / kParts)-th of them.  Whole records per wave, rather than parts
+  // dealt out one by one, bound the parts with candidates that one wave
+  // checks (a record's candidates are usually in one part).  The record of
+  // the item after next is fetched, and the next item's loads are issued,
+  // while this one is checked.
   constexpr int kRq = kResolveQt<QT>, kParts = QT / kRq;
-  const uint32_t n_items = n_recs * kParts;
-  const uint32_t stride = gridDim.x * kResolveWaves;
-  uint32_t r = blockIdx.x * kResolveWaves + wave;
-  // the next record is fetched while this one runs
-  uint4 rec = r < n_items ? recs[r / kParts] : make_uint4(0, 0, 0, 0);
-  for (; r < n_items; r += stride) {
-    const uint4 next = r + stride < n_items ? recs[(r + stride) / kParts] : make_uint4(0, 0, 0, 0);
-    const uint64_t row0 = ((uint64_t)rec.y << 32 | rec.x) * kSharedRows;
-    resolve_record<kRq, kPlane>(L, db, plane, n_rows, row0, rec.z + (r % kParts) * (kRq * 16),
-                                part_groups<kParts>(rec.w, r % kParts), queries, n_queries, threshold, hits,
-                                capacity, counter, index_base);
-    rec = next;
+  const uint32_t w0 = blockIdx.x * kResolveWaves + wave;
+  const uint32_t S = gridDim.x * kResolveWaves;
+  auto rec_of = [&](uint32_t j) { return w0 + (j / kParts) * S; };
+  auto fetch = [&](uint32_t j) {
+    const uint32_t i = rec_of(j);
+    return i < n_recs ? recs[i] : make_uint4(0, 0, 0, 0);
+  };
+  auto row0_of = [](const uint4& x) { return ((uint64_t)x.y << 32 | x.x) * kSharedRows; };
+  // the pending groups of item j (none past the last record)
+  auto pending_of = [&](const uint4& x, uint32_t j) {
+    return rec_of(j) < n_recs ? part_groups<kParts>(x.w, j % kParts) : 0u;
+  };
+  auto qbase_of = [](const uint4& x, uint32_t j) { return x.z + (j % kParts) * (kRq * 16); };
+  uint32_t j = 0;
+  uint4 rec = fetch(0);
+  uint4 rec_next = fetch(1);
+  PartLoads<kRq> P;
+  part_load<kRq, kPlane>(P, db, plane, n_rows, row0_of(rec), qbase_of(rec, 0), pending_of(rec, 0), queries,
+                         n_queries);
+  // The first part's words arrive here (vmcnt 0; expcnt and lgkmcnt left
+  // alone), as every later part's do at the end of the previous iteration:
+  // the compiler then sees them complete on every path into the loop and
+  // does not make the loop's waits for them cover the next part's loads.
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  for (; rec_of(j) < n_recs; j++) {
+    uint4 rec_after;
+    PartLoads<kRq> Pn;
+    resolve_record<kRq, kPlane>(L, P, db, plane, n_rows, row0_of(rec), qbase_of(rec, j), pending_of(rec, j),
+                                queries, n_queries, threshold, hits, capacity, counter, index_base, [&] {
+                                  rec_after = fetch(j + 2);
+                                  part_load<kRq, kPlane>(Pn, db, plane, n_rows, row0_of(rec_next),
+                                                         qbase_of(rec_next, j + 1), pending_of(rec_next, j + 1),
+                                                         queries, n_queries);
+                                });
+    rec = rec_next;
+    rec_next = rec_after;
+    P = Pn;
   }
   resolve_flush(L, hits, capacity, counter);
 }
@@ -1012,20 +1125,27 @@ __global__ void __launch_bounds__(64 * kResolveWaves) pdq_match_resolve_shards_k
     }
     return;
   }
+  // whole records per wave, as in pdq_match_resolve_kernel
   constexpr int kRq = kResolveQt<QT>, kParts = QT / kRq;
-  const uint32_t n_items = n_recs * kParts;
   const uint32_t stride = gridDim.x * kResolveWaves;
-  uint32_t r = blockIdx.x * kResolveWaves + wave;
-  uint4 rec = r < n_items ? recs[r / kParts] : make_uint4(0, 0, 0, 0);
-  for (; r < n_items; r += stride) {
-    const uint4 next = r + stride < n_items ? recs[(r + stride) / kParts] : make_uint4(0, 0, 0, 0);
+  uint32_t i = blockIdx.x * kResolveWaves + wave;
+  uint4 rec = i < n_recs ? recs[i] : make_uint4(0, 0, 0, 0);
+  for (; i < n_recs; i += stride) {
+    const uint4 next = i + stride < n_recs ? recs[i + stride] : make_uint4(0, 0, 0, 0);
     const uint64_t gsub = (uint64_t)rec.y << 32 | rec.x;
     const ShardView& v = shards[shard_of_chunk(shards, n_shards, gsub / kSubsPerChunk)];
     const uint64_t row0 = (gsub - v.chunk_begin * kSubsPerChunk) * kSharedRows;
-    resolve_record<kRq, kPlane>(L, reinterpret_cast<const uint32_t*>(v.rows),
-                                reinterpret_cast<const uint32_t*>(v.plane), v.n_rows, row0,
-                                rec.z + (r % kParts) * (kRq * 16), part_groups<kParts>(rec.w, r % kParts), queries,
-                                nq, threshold, hits, capacity, counter, v.index_base);
+    const uint32_t* rows = reinterpret_cast<const uint32_t*>(v.rows);
+    const uint32_t* vplane = reinterpret_cast<const uint32_t*>(v.plane);
+#pragma unroll 1
+    for (int part = 0; part < kParts; part++) {
+      const uint32_t qbase = rec.z + part * (kRq * 16);
+      const uint32_t pending = part_groups<kParts>(rec.w, part);
+      PartLoads<kRq> P;
+      part_load<kRq, kPlane>(P, rows, vplane, v.n_rows, row0, qbase, pending, queries, nq);
+      resolve_record<kRq, kPlane>(L, P, rows, vplane, v.n_rows, row0, qbase, pending, queries, nq, threshold, hits,
+                                  capacity, counter, v.index_base, [] {});
+    }
     rec = next;
   }
   resolve_flush(L, hits, capacity, counter);
