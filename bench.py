@@ -361,8 +361,12 @@ def main():
     # Hash algorithmic bytes: the u8 plane (262,144 B) + 8 x (32 + 4 + 4) B out.
     hash_bytes = (262144.0 + 320.0) * B
     hash_gbs = hash_bytes / (t_hash_head * 1e-3) / 1e9
-    layout_head = "pdq_hash_kernel<u8> (one wave per image)" if B >= 1024 else \
-        "pdq_hash_block_kernel<u8> (eight waves per image)"
+    # the automatic layout: a wave per image past 1024 images, sixteen waves
+    # while the images fit one per CU, else eight
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    layout_head = ("pdq_hash_kernel<u8> (one wave per image)" if B > 1024 else
+                   "pdq_hash_block_kernel<u8> (sixteen waves per image)" if B <= cus else
+                   "pdq_hash_block_kernel<u8> (eight waves per image)")
     roofline_hash = {
         "kernel": f"{layout_head}: {B} images per launch",
         "bound": "hbm", "achieved": round(hash_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
