@@ -285,6 +285,48 @@ def test_counter_reset_without_append(gpu, oracle, dev, nq):
     assert gpu.match(empty, d_q, 31, count=count, path="mfma")[0] == 0
 
 
+def test_record_count_slots_alternate(gpu, oracle, dev):
+    """Shared-rows calls on one stream take the record area's two count slots
+    in turn: each call's streaming kernel clears the slot of the stream's
+    next call and, without the append flag, the caller's hit counter, so only
+    a stream's first call (or one after a failed launch) needs a clearing
+    launch.  A sequence of shared-rows calls mixed with per-wave calls (64
+    queries), an append, garbage counters and a call whose records overflow
+    the cap stays exact call after call."""
+    import torch
+    db = synthetic.random_db(20011, seed=80)
+    q = synthetic.random_db(1024, seed=81)
+    synthetic.plant(db, q, 300, seed=82)
+    d_db, d_q = torch.from_numpy(db).to(dev), torch.from_numpy(q).to(dev)
+    exp_n, exp = oracle.match_all(db, q, 31)
+    exp64_n, exp64 = oracle.match_all(db, q[:64], 31)
+    plane = gpu.build_prefix_plane(d_db)
+    stream = torch.cuda.Stream()
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    steps = ["shared", "shared", "append", "per-wave", "shared", "overflow", "shared", "shared", "per-wave", "shared"]
+    with torch.cuda.stream(stream):
+        for i, kind in enumerate(steps):
+            say(f"[match] count slots: step {i} {kind}")
+            append = kind == "append"
+            if not append:
+                count.fill_(987654321)
+            if kind == "overflow":
+                gpu.set_match_record_cap(8)
+            try:
+                queries = d_q[:64] if kind == "per-wave" else d_q
+                total, hits = gpu.match(d_db, queries, 31, capacity=1 << 16, count=count, path="mfma",
+                                        plane=plane if i % 2 else None, append=append, stream=stream)
+            finally:
+                if kind == "overflow":
+                    gpu.set_match_record_cap(0)
+            if kind == "per-wave":
+                assert total == exp64_n and np.array_equal(sort_triples(as_triples(hits)), exp64)
+            elif append:
+                assert total == 2 * exp_n
+            else:
+                assert total == exp_n and np.array_equal(sort_triples(as_triples(hits)), exp)
+
+
 def test_index_base_and_append(gpu, oracle, dev):
     """Two shards into one hit buffer with global indices (sharded DB)."""
     import torch

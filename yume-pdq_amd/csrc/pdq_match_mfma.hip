@@ -727,13 +727,21 @@ __device__ __forceinline__ void item_coords(uint64_t item, uint32_t groups, uint
 // One launch, one block per work item: row chunks x query groups (the
 // per-wave layout, which checks its candidates itself), or, with kShared
 // (4+ query groups), 1024-row sub-chunks x groups of kMfmaWaves query groups,
-// whose candidates become records for pdq_match_resolve_kernel.
+// whose candidates become records for pdq_match_resolve_kernel.  Block 0
+// first stores zero to clear_rec and clear_hits where they are given: the
+// record count of the stream's next call, and (shared rows, which append
+// no hits here) this call's hit counter.
 template <int QT, bool kPlane, bool kShared, bool kTri = false>
 __global__ void __launch_bounds__(64 * kMfmaWaves) __attribute__((amdgpu_waves_per_eu(QT >= 16 ? 3 : 4))) pdq_match_mfma_kernel(
     const uint32_t* __restrict__ db, const uint32_t* __restrict__ plane, uint64_t n_rows,
     const uint32_t* __restrict__ queries, uint32_t n_queries, uint32_t threshold, MatchHit* __restrict__ hits,
     uint64_t capacity, unsigned long long* __restrict__ counter, uint64_t index_base, uint32_t groups,
-    uint4* __restrict__ recs, uint32_t* __restrict__ rec_count, uint32_t rec_cap) {
+    uint4* __restrict__ recs, uint32_t* __restrict__ rec_count, uint32_t rec_cap, uint32_t* __restrict__ clear_rec,
+    unsigned long long* __restrict__ clear_hits) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (clear_rec) *clear_rec = 0u;
+    if (clear_hits) *clear_hits = 0ull;
+  }
   __shared__ uint32_t lut[256];
   init_lut(lut);
   uint64_t chunk;
@@ -1211,15 +1219,24 @@ bool set_match_record_cap(uint32_t cap) {
   return true;
 }
 
+// A stream's record area: the records, then two record-count slots
+// (kCountSlotStride words apart).  Calls on the stream take the slots in
+// turn, and each call's first kernel clears the slot of the next call, so a
+// call needs no launch of its own to clear its count while the previous
+// call's kernels were all enqueued.  slot and slot_clear are host state.
+constexpr uint32_t kCountSlotStride = 32;
+static_assert(2 * kCountSlotStride * sizeof(uint32_t) <= 256, "both count slots in the area's 256-byte tail");
 struct RecordArea {
   uint4* recs = nullptr;
   uint32_t* count = nullptr;
   int cus = 0;  // compute units of the area's device
+  uint32_t slot = 0;        // the count slot of the stream's next call
+  bool slot_clear = false;  // that slot holds zero once the enqueued work has run
 };
 
 static std::mutex g_record_mu;
 
-static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable) {
+static hipError_t record_area(hipStream_t stream, RecordArea** out, bool* usable) {
   *usable = false;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   hipError_t e = hipStreamIsCapturing(stream, &cs);
@@ -1244,7 +1261,7 @@ static hipError_t record_area(hipStream_t stream, RecordArea* out, bool* usable)
     it = areas.emplace(std::make_pair(dev, stream), ra).first;
     per_device[dev]++;
   }
-  *out = it->second;
+  *out = &it->second;
   *usable = true;
   return hipSuccess;
 }
@@ -1311,13 +1328,15 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
     // sized for this call's worst case), else this stream's own area
     std::lock_guard<std::mutex> g(g_record_mu);
     RecordArea ra;
+    RecordArea* area = nullptr;  // the stream's own area, whose count slots alternate
     bool usable = false;
     if (fixed) {
       ra = *fixed;
       usable = true;
     } else {
-      hipError_t e = record_area(stream, &ra, &usable);
+      hipError_t e = record_area(stream, &area, &usable);
       if (e != hipSuccess) return e;
+      if (usable) ra = *area;
     }
     if (usable) {
       const uint32_t cap = fixed ? kScratchRecords : record_cap();
@@ -1325,38 +1344,58 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
       const uint32_t sgroups = (groups + kMfmaWaves - 1) / kMfmaWaves;
       const uint64_t blocks = subs8 * sgroups;
       if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-      hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, ra.count, reset ? counter : nullptr);
+      // this call's count slot; the stream's area has the streaming kernel
+      // clear the next call's slot, and the hit counter when this slot is
+      // already clear
+      uint32_t* rc = ra.count;
+      uint32_t* rc_next = nullptr;
+      unsigned long long* hits_clear = nullptr;
+      if (area) {
+        rc = area->count + kCountSlotStride * area->slot;
+        rc_next = area->count + kCountSlotStride * (area->slot ^ 1u);
+      }
+      if (area && area->slot_clear)
+        hits_clear = reset ? counter : nullptr;
+      else
+        hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(1), 0, stream, rc, reset ? counter : nullptr);
       // three products per accumulator where they apply
       const bool tri = QT == 16 && threshold >= kTriMinThreshold;
       if (plane && tri)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true, QT == 16>), dim3((unsigned)blocks), block, 0,
                            stream, d, p, n_rows, q, n_queries, threshold, hits, capacity, counter, index_base,
-                           sgroups, ra.recs, ra.count, cap);
+                           sgroups, ra.recs, rc, cap, rc_next, hits_clear);
       else if (plane)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
-                           ra.count, cap);
+                           rc, cap, rc_next, hits_clear);
       else if (tri)
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true, QT == 16>), dim3((unsigned)blocks), block, 0,
                            stream, d, p, n_rows, q, n_queries, threshold, hits, capacity, counter, index_base,
-                           sgroups, ra.recs, ra.count, cap);
+                           sgroups, ra.recs, rc, cap, rc_next, hits_clear);
       else
         hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, true>), dim3((unsigned)blocks), block, 0, stream, d, p,
                            n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, sgroups, ra.recs,
-                           ra.count, cap);
+                           rc, cap, rc_next, hits_clear);
       // the resident blocks of the recheck kernel; the per-wave rerun after
       // an overflow walks all items with them
       const uint64_t rgrid = std::min<uint64_t>(match_resolve_blocks(ra.cus), chunks8 * groups);
       const dim3 rblock(64 * kResolveWaves);
       if (plane)
         hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, true>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
-                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
-                           cap, chunks8, groups);
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, rc, cap,
+                           chunks8, groups);
       else
         hipLaunchKernelGGL((pdq_match_resolve_kernel<QT, false>), dim3((unsigned)rgrid), rblock, 0, stream, d, p,
-                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, ra.count,
-                           cap, chunks8, groups);
-      return hipGetLastError();
+                           n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, ra.recs, rc, cap,
+                           chunks8, groups);
+      const hipError_t e = hipGetLastError();
+      if (area) {
+        // the next call's slot is clear only if this call's kernels were
+        // all enqueued; otherwise the next call clears it with a launch
+        if (e == hipSuccess) area->slot ^= 1u;
+        area->slot_clear = e == hipSuccess;
+      }
+      return e;
     }
   }
   const uint64_t blocks = chunks8 * groups;
@@ -1365,11 +1404,11 @@ static hipError_t launch_qt(const void* db, const void* plane, uint64_t n_rows, 
   if (plane)
     hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, true, false>), dim3((unsigned)blocks), block, 0, stream, d, p,
                        n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, groups, nullptr, nullptr,
-                       0u);
+                       0u, nullptr, nullptr);
   else
     hipLaunchKernelGGL((pdq_match_mfma_kernel<QT, false, false>), dim3((unsigned)blocks), block, 0, stream, d, p,
                        n_rows, q, n_queries, threshold, hits, capacity, counter, index_base, groups, nullptr, nullptr,
-                       0u);
+                       0u, nullptr, nullptr);
   return hipGetLastError();
 }
 
