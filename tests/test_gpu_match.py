@@ -327,6 +327,47 @@ def test_record_count_slots_alternate(gpu, oracle, dev):
                 assert total == exp_n and np.array_equal(sort_triples(as_triples(hits)), exp)
 
 
+def test_explicit_stream_and_per_thread_streams(gpu, oracle, dev):
+    """match() on a stream that is not torch's current one waits for that
+    stream before it reads the count; and two host threads matching at once
+    on hipStreamPerThread (one handle, a different stream in each thread) get
+    record areas of their own, so both stay exact call after call."""
+    import threading
+
+    import torch
+    db = synthetic.random_db(30011, seed=90)
+    qa = synthetic.random_db(1024, seed=91)
+    qb = synthetic.random_db(1024, seed=92)
+    synthetic.plant(db, qa, 200, seed=93)
+    synthetic.plant(db, qb, 200, seed=94)
+    d_db = torch.from_numpy(db).to(dev)
+    d_qa, d_qb = torch.from_numpy(qa).to(dev), torch.from_numpy(qb).to(dev)
+    exp = {0: oracle.match_all(db, qa, 31), 1: oracle.match_all(db, qb, 31)}
+    say("[match] explicit stream, read outside its context")
+    s = torch.cuda.Stream()
+    total, hits = gpu.match(d_db, d_qa, 31, path="mfma", stream=s)
+    assert total == exp[0][0] and np.array_equal(sort_triples(as_triples(hits)), exp[0][1])
+    torch.cuda.synchronize()
+    per_thread = torch.cuda.ExternalStream(2)  # hipStreamPerThread
+    results = {0: [], 1: []}
+
+    def worker(k):
+        for _ in range(4):
+            n, h = gpu.match(d_db, d_qa if k == 0 else d_qb, 31, path="mfma", stream=per_thread)
+            results[k].append((n, sort_triples(as_triples(h))))
+
+    say("[match] two threads on hipStreamPerThread")
+    threads = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    for k in (0, 1):
+        assert len(results[k]) == 4
+        for n, t in results[k]:
+            assert n == exp[k][0] and np.array_equal(t, exp[k][1])
+
+
 def test_index_base_and_append(gpu, oracle, dev):
     """Two shards into one hit buffer with global indices (sharded DB)."""
     import torch

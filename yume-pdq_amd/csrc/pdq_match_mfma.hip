@@ -26,6 +26,8 @@
 #include <atomic>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <tuple>
 #include <utility>
 
 #include "pdq_device.h"
@@ -1244,9 +1246,13 @@ static hipError_t record_area(hipStream_t stream, RecordArea** out, bool* usable
   if (cs != hipStreamCaptureStatusNone) return hipSuccess;
   int dev = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-  static std::map<std::pair<int, hipStream_t>, RecordArea> areas;  // guarded by g_record_mu
-  static std::map<int, size_t> per_device;                          // areas made per device
-  auto it = areas.find({dev, stream});
+  // keyed by device and stream; hipStreamPerThread names a different
+  // stream in each host thread, so its areas are also keyed by the thread
+  using Key = std::tuple<int, hipStream_t, std::thread::id>;
+  static std::map<Key, RecordArea> areas;  // guarded by g_record_mu
+  static std::map<int, size_t> per_device;  // areas made per device
+  const Key key{dev, stream, stream == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id()};
+  auto it = areas.find(key);
   if (it == areas.end()) {
     if (per_device[dev] >= kMaxStreamAreas) return hipSuccess;  // not usable: the per-wave layout
     void* p = nullptr;
@@ -1258,7 +1264,7 @@ static hipError_t record_area(hipStream_t stream, RecordArea** out, bool* usable
     }
     ra.recs = static_cast<uint4*>(p);
     ra.count = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(p) + (size_t)kRecordCap * sizeof(uint4));
-    it = areas.emplace(std::make_pair(dev, stream), ra).first;
+    it = areas.emplace(key, ra).first;
     per_device[dev]++;
   }
   *out = &it->second;

@@ -420,6 +420,8 @@ def match(db, queries, threshold, capacity=1 << 16, index_base=0, stream=None, h
             capacity, C.c_void_p(count.data_ptr()), index_base, flags, _stream_ptr(stream)), "match")
     if not sync:
         return hits, count
+    if stream is not None:
+        stream.synchronize()  # the reads below run on torch's current stream
     total = int(count.item())
     k = min(total, capacity)
     arr = hits[:k].cpu().numpy().copy().view(HIT_DTYPE).reshape(-1)
@@ -522,6 +524,8 @@ def match_first(db, queries, threshold, stream=None):
     _check(lib().yume_pdq_amd_match_first(
         C.c_void_p(db.data_ptr()), db.shape[0], C.c_void_p(queries.data_ptr()), queries.shape[0],
         int(threshold), C.c_void_p(first.data_ptr()), _stream_ptr(stream)), "match_first")
+    if stream is not None:
+        stream.synchronize()  # the read below runs on torch's current stream
     key = int(first.item()) & 0xFFFFFFFFFFFFFFFF
     if key == 0xFFFFFFFFFFFFFFFF:
         return None
