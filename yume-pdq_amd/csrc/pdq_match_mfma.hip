@@ -972,10 +972,9 @@ __device__ __forceinline__ void resolve_record(ResolveLds& L, const PartLoads<QT
 
 // Query tiles per wave in the recheck: at most eight.  A record of a 16-tile
 // stream is checked as two halves (query tiles 0-7 and 8-15, each with its
-// own pending groups) by two waves: sixteen tiles' operands need more
-// registers than two blocks per CU leave, and a record's check is a chain of
-// dependent steps whose length, not the total work, sets the kernel's time
-// when records are few.
+// own pending groups), one after the other by the wave that holds the
+// record: sixteen tiles' operands need more registers than two blocks per CU
+// leave, and a half without candidates costs nothing.
 template <int QT>
 constexpr int kResolveQt = QT > 8 ? 8 : QT;
 

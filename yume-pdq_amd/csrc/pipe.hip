@@ -4,7 +4,8 @@
 // (src/main.rs:1091-1304, record layout src/main.rs:662-773).
 //
 // Pipeline: a reader thread fills a ring of page-locked batch buffers with
-// whole frames; the processing thread uploads a batch, runs the fused hash
+// whole frames; the processing thread uploads a batch (in chunks of 64
+// frames, each as soon as the reader has read it), runs the fused hash
 // kernel (input conversion fused into its row loader) and reads the hashes,
 // qualities and thresholds back, and while the GPU works on batch i + 1 it
 // formats and writes batch i.
@@ -240,10 +241,18 @@ bool write_full(int fd, const char* src, size_t len) {
 struct Slot {
   uint8_t* frames = nullptr;  // page-locked, batch x frame bytes
   uint32_t n = 0;             // whole frames read
+  uint32_t ready = 0;         // frames of the current fill read so far (0 once released)
   int status = 0;             // 0, or the error that ended the stream after these frames
   bool last = false;          // the stream ends after this slot
   bool full = false;          // filled by the reader, not yet released
 };
+
+// The reader publishes a slot's progress every kUploadChunk frames and the
+// processing thread uploads each chunk as it is published, so a batch of
+// 256 frames (64 MiB) leaves for the GPU in 16 MiB pieces right behind the
+// reader instead of after it: batches of 256 then run as fast as batches of
+// 64 (whose upload is one piece, as before).
+constexpr uint32_t kUploadChunk = 64;
 
 struct Outputs {
   uint8_t* d_hashes = nullptr;
@@ -421,6 +430,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
   Slot* slots = res->slots;
   for (int s = 0; s < kSlots; s++) {
     slots[s].n = 0;
+    slots[s].ready = 0;
     slots[s].status = 0;
     slots[s].last = false;
     slots[s].full = false;
@@ -462,10 +472,18 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
             break;
           }
           n++;
+          if (n % kUploadChunk == 0 && n < B) {
+            {
+              std::lock_guard<std::mutex> lk(mu);
+              s.ready = n;
+            }
+            cv.notify_all();
+          }
         }
         {
           std::lock_guard<std::mutex> lk(mu);
           s.n = n;
+          s.ready = n;
           s.status = status;
           s.last = last;
           s.full = true;
@@ -493,6 +511,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     {
       std::lock_guard<std::mutex> lk(mu);
       slots[pending].full = false;
+      slots[pending].ready = 0;
     }
     cv.notify_all();
     pending = -1;
@@ -525,11 +544,33 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
   for (uint64_t k = 0; rc == 0 && reader.joinable(); k++) {
     const int si = (int)(k % kSlots);
     Slot& s = slots[si];
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [&] { return s.full; });
-    }
     const int oi = (int)(k & 1);
+    // the slot's frames go up in chunks as the reader publishes them, until
+    // the slot is full.  d_in[oi] and outs[oi] were last used two batches
+    // ago, whose done event the host has already waited for
+    hipError_t up_err = hipSuccess;
+    for (uint32_t up = 0;;) {
+      uint32_t avail = 0;
+      bool full = false;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return s.full || s.ready >= up + kUploadChunk; });
+        full = s.full;
+        avail = full ? s.n : s.ready;
+      }
+      if (avail > up) {
+        up_err = hipMemcpyAsync(static_cast<uint8_t*>(d_in[oi]) + (uint64_t)up * frame_bytes,
+                                s.frames + (uint64_t)up * frame_bytes, (uint64_t)(avail - up) * frame_bytes,
+                                hipMemcpyHostToDevice, upload);
+        up = avail;
+      }
+      if (full || up_err != hipSuccess) break;
+    }
+    if (up_err != hipSuccess) {
+      hip_failed = true;
+      rc = fail_hip(up_err);
+      break;
+    }
     if (s.n > 0) {
       Outputs& x = outs[oi];
       HashOutputs ho;
@@ -540,10 +581,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
       ho.buf1 = nullptr;
       ho.tmp = nullptr;
       ho.dihedral_pdqf = diag ? x.d_pdqf : nullptr;
-      // d_in[oi] and outs[oi] were last used two batches ago, whose done
-      // event the host has already waited for
-      hipError_t err = hipMemcpyAsync(d_in[oi], s.frames, s.n * frame_bytes, hipMemcpyHostToDevice, upload);
-      if (err == hipSuccess) err = hipEventRecord(x.uploaded, upload);
+      hipError_t err = hipEventRecord(x.uploaded, upload);
       if (err == hipSuccess) err = hipStreamWaitEvent(stream, x.uploaded, 0);
       if (err == hipSuccess) err = launch_hash(d_in[oi], o.input_kind, frame_bytes, s.n, o.dihedral_mode, ho, stream);
       if (err == hipSuccess) {
@@ -571,6 +609,7 @@ int run_pipe(const PipeOptions& o, uint64_t* frames_out, std::string* error) {
     } else {
       std::lock_guard<std::mutex> lk(mu);
       s.full = false;
+      s.ready = 0;
     }
     if (s.last) {
       if ((rc = drain()) != 0) break;
